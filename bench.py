@@ -1,0 +1,198 @@
+"""Throughput benchmark: ray-steps/s on the 512^3 turbulent box with absorption.
+
+Workload (one GPU): 512^3 fp32 grid over a 20 mm cube, n_e = max(0, 2e24 (1 + 0.3 g))
+with g a unit-variance Kolmogorov (k^-11/3) Gaussian random field between
+k = 2 pi / 20 mm and 2 pi / 0.2 mm; Te = 20 eV, Z = 1, Coulomb log 10;
+1e7 rays at 532 nm from a seeded collimated square beam (half-width 8 mm)
+entering at z = -10 mm; fixed-step RK4 at half a cell; integration horizon
+sqrt(8) * 20 mm / c (the reference ``solve`` horizon).
+
+One step = one pass of the traced path over the batch: Morton keys + sort,
+RK4 trace (FP64 tau and path), Jones epilogue with exp(-tau), and a 256 x 256
+detector histogram.  The packed grid and the rays are resident in HBM before
+timing starts.  value = in-box ray-steps per second (a ray-step is one RK4
+step with a stage point inside the grid box).
+
+Prints one JSON line (rank 0).  Also reports the roofline of the trace kernel
+(algorithmic bytes = 4 RHS x 8 corners x 4 channels x 4 B = 512 B per
+ray-step, over the kernel's average duration from HIP events on its stream)
+and a CPU baseline: the FP64 C oracle (OpenMP) on a bounded subsample of the
+same rays, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+C = 299792458.0
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_STEP = 4 * 8 * 4 * 4  # 4 RHS x 8 corners x 4 fp32 channels
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per trace launch measured with rocprofv3 --pmc (committed summary), or None."""
+    p = os.path.join(ROOT, "profiles", "trace_kernel_turbulent512_pmc.md")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(ne, coords, s0, nsub, h, omega, target_s):
+    from oracle import oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    c64 = [c.astype(np.float64) for c in coords]
+    ne64 = ne.astype(np.float64)
+    grads = O.gradients(ne64, *c64, omega)
+    kap = O.kappa(ne64, 20.0, 1.0, omega, coulomb_log=10.0)
+    h64 = np.asarray(h, dtype=np.float32).astype(np.float64)
+    rng = np.random.default_rng(1)
+    # calibrate on 2000 rays, then size the sample for ~target_s seconds
+    pick = rng.choice(s0.shape[1], 2000, replace=False)
+    t0 = time.perf_counter()
+    _, st = O.trace(grads, kap, c64, s0[:, pick].astype(np.float64), nsub, h64, nthreads=cores)
+    dt = time.perf_counter() - t0
+    rate = st.sum() / dt
+    n = int(min(s0.shape[1], max(2000, target_s * rate / max(st.mean(), 1))))
+    pick = rng.choice(s0.shape[1], n, replace=False)
+    t0 = time.perf_counter()
+    _, st = O.trace(grads, kap, c64, s0[:, pick].astype(np.float64), nsub, h64, nthreads=cores)
+    dt = time.perf_counter() - t0
+    del grads, kap
+    return {"value": float(st.sum() / dt), "unit": "ray-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} of the workload's rays (seeded random subset), same grid and "
+                      f"schedule, FP64 C oracle with OpenMP, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--rays", type=int, default=10_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the traced path runs on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from sloth_amd.core import engine
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import kolmogorov_field
+    from sloth_amd.processing.diagnostics import histogram2d_device
+    from sloth_amd.shared.propagation import jones_device
+
+    dev = torch.device("cuda", 0)
+    L = 20e-3
+    n = args.grid
+    lwl = 532e-9
+    omega = 2 * np.pi * C / lwl
+    log(f"[bench] field {n}^3 ...")
+    ne, coords = kolmogorov_field(n, L, n0=2e24, frac=0.3, l_max=L, l_min=0.2e-3, seed=0,
+                                  device=dev)
+    log("[bench] beam ...")
+    s0_host = Beam(args.rays, 8e-3, 0.0, L / 2, beam_type="square", seeded=True).s0
+    grid = engine.PackedGrid(ne, *coords, omega, inv_brems=True, Te=20.0, Z=1.0,
+                             coulomb_log=10.0)
+    s0 = torch.from_numpy(s0_host).to(dev)
+    T = np.sqrt(8.0) * L / C
+    save_times = np.array([0.0, T])
+    edges = np.linspace(-L / 2, L / 2, 257)
+    stream = torch.cuda.current_stream()
+
+    def step(events=None):
+        order = engine.ray_order(grid, s0)
+        if events is not None:
+            events[0].record(stream)
+        res = engine.trace(grid, s0, save_times, save_all=False, want_path=True, order=order)
+        if events is not None:
+            events[1].record(stream)
+        rf, amp = jones_device(res.states[0], "z", res.tau[0])
+        counts, wsum = histogram2d_device(rf[0], rf[2], edges, edges, weights=amp)
+        return res, counts, wsum
+
+    log("[bench] warmup ...")
+    for _ in range(args.warmup):
+        res, counts, _ = step()
+    torch.cuda.synchronize()
+    total_steps = int(res.total_steps.item())
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    log(f"[bench] timing {args.steps} steps ({total_steps} ray-steps each) ...")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        res, counts, _ = step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert int(res.total_steps.item()) == total_steps
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    ms_per_step = wall * 1e3 / args.steps
+    value = total_steps / (ms_per_step * 1e-3)
+    achieved = total_steps * BYTES_PER_STEP / (kern_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(),
+                "kernel": "slt::trace_kernel<true,true>", "kernel_ms": round(kern_ms, 3),
+                "algorithmic_bytes_per_launch": total_steps * BYTES_PER_STEP}
+    log(f"[bench] {ms_per_step:.2f} ms/step, trace kernel {kern_ms:.2f} ms, "
+        f"{value:.3e} ray-steps/s, detector sum {int(counts.sum())}")
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        del s0, res
+        torch.cuda.empty_cache()
+        nsub, h = engine.schedule(save_times, engine.nominal_step(grid, 0.5))
+        cpu = cpu_baseline(ne, coords, s0_host, nsub, h, omega, args.cpu_seconds)
+        log(f"[bench] cpu {cpu['value']:.3e} ray-steps/s on {cpu['cores']} cores")
+    out = {
+        "metric": "ray-steps/s (512^3 grid, 1e7 rays)",
+        "value": value,
+        "unit": "ray-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"turbulent_box_{n}_absorption", "grid": [n, n, n],
+                   "box_mm": 20, "rays": args.rays, "lambda_nm": 532, "beam": "square 8 mm half-width",
+                   "Te_eV": 20, "Z": 1, "coulomb_log": 10, "integrator": "rk4, 0.5 cell",
+                   "horizon": "sqrt(8) L / c", "ray_steps_per_pass": total_steps,
+                   "accumulators": "fp64 tau and path"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

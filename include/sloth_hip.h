@@ -1,0 +1,181 @@
+/*
+ * sloth_hip.h -- C ABI of libsloth_hip.so, the MI355X (gfx950) ray tracer that
+ * replaces the hot path of MAGPIE-ICL/SLOTH.
+ *
+ * Conventions
+ *   - Every device pointer is caller-owned (e.g. a PyTorch tensor's data_ptr()).
+ *     The library never frees caller memory and allocates none on these paths.
+ *   - Every call is ordered on the given HIP stream (NULL = default stream) and
+ *     returns asynchronously; 0 = success, < 0 = error (message in
+ *     slt_last_error(), thread-local).  No C++ exception crosses this ABI.
+ *   - Calls on distinct streams with distinct output buffers may run
+ *     concurrently.
+ *
+ * The reference is pure Python/JAX: there is no FFI in it.  Each entry point
+ * below names the reference function whose work it takes over; the Python
+ * mirror in sloth_amd/ binds them with ctypes (INTEGRATION.md shows the stub).
+ */
+#ifndef SLOTH_HIP_H
+#define SLOTH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLT_ABI_VERSION 1
+
+#define SLT_OK 0
+#define SLT_ERR_INVALID (-1)     /* bad argument (shape, pointer, range) */
+#define SLT_ERR_HIP (-2)         /* HIP runtime error (launch, memory)     */
+#define SLT_ERR_UNSUPPORTED (-3) /* valid input outside what the kernels handle */
+
+typedef void* slt_stream_t; /* hipStream_t */
+
+/*
+ * Layout of a packed grid: one 16-byte voxel (d/dx, d/dy, d/dz of the
+ * refraction driving term, kappa) per grid point, addressed as
+ *   voxel(i, j, k) = base + i*stride[0] + j*stride[1] + k*stride[2]
+ * where (i, j, k) index the x, y, z coordinate arrays.  The probing axis is
+ * laid out fastest so that consecutive steps of a ray reuse cache lines.
+ */
+typedef struct slt_grid_desc {
+    int32_t n[3];      /* points per axis (x, y, z), each >= 2           */
+    int32_t nchan;     /* 3: gradient only; 4: gradient + kappa          */
+    int64_t stride[3]; /* voxel strides (units of 16-byte voxels)        */
+    float lo[3];       /* first coordinate value x[0], y[0], z[0] (fp32) */
+    float hi[3];       /* last coordinate value x[n-1], ...      (fp32)  */
+    float inv_d[3];    /* (n-1) / (hi - lo)                              */
+    int32_t probing_axis; /* 0, 1, 2 for x, y, z                         */
+} slt_grid_desc;
+
+/* Library ABI version (SLT_ABI_VERSION). */
+int slt_version(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char* slt_last_error(void);
+
+/*
+ * Host-only: fill *out from the fp32 coordinate arrays (host memory).  The
+ * kernels use uniform index arithmetic, so coordinates must be a uniformly
+ * spaced ascending sequence up to fp32 rounding (any linspace is); otherwise
+ * SLT_ERR_UNSUPPORTED.  Replaces the coordinate handling of
+ * src/core/interpolator.py:72-87 (per-axis searchsorted).
+ */
+int slt_grid_desc_init(const float* x, const float* y, const float* z, int32_t nx, int32_t ny,
+                       int32_t nz, int32_t probing_axis, int32_t with_kappa,
+                       slt_grid_desc* out);
+
+/* Bytes of the packed voxel buffer described by *g. */
+int64_t slt_grid_bytes(const slt_grid_desc* g);
+
+/*
+ * k1: gradient stencil + inverse-bremsstrahlung rate + 16-byte voxel pack.
+ * Replaces precompute_gradients (src/core/propagator.py:39-93) and
+ * kappa_inv_brems (src/core/propagator.py:116-164).
+ *   ne            device fp32 (nx, ny, nz) C-order, electron density [m^-3]
+ *   cx, cy, cz    device fp32 coordinate arrays (the gradient uses the actual
+ *                 coordinates, as numpy/jnp.gradient with coordinate arrays does)
+ *   grad_coeff    -0.5 c^2 / (3.14207787e-4 omega^2), computed in fp64
+ *   te, n_te      device fp32 electron temperature [eV], n_te = 1 or nx*ny*nz
+ *   zz, n_z       device fp32 mean charge state,      n_z  = 1 or nx*ny*nz
+ *                 (te / zz may be NULL when kappa is not wanted)
+ *   omega, e_charge, epsilon_0, speed_of_light   physical constants (fp64)
+ *   coulomb_log   > 0 overrides the Coulomb logarithm (extension); <= 0 uses
+ *                 the reference's max(2, ln(v_the / (omega_max b_min)))
+ *   packed        device voxel buffer (slt_grid_bytes(g) bytes) or NULL
+ *   planar[4]     optional device fp32 (nx,ny,nz) C-order outputs for
+ *                 d/dx, d/dy, d/dz, kappa (each may be NULL)
+ * Arithmetic is fp64; results are rounded once to fp32.
+ */
+int slt_grid_pack(const float* ne, const float* cx, const float* cy, const float* cz,
+                  double grad_coeff, const float* te, int64_t n_te, const float* zz,
+                  int64_t n_z, double omega, double e_charge, double epsilon_0,
+                  double speed_of_light, double coulomb_log, const slt_grid_desc* g,
+                  void* packed, float* planar_dx, float* planar_dy, float* planar_dz,
+                  float* planar_kappa, slt_stream_t stream);
+
+/*
+ * Element-wise inverse-bremsstrahlung rate [1/s] of n densities (fp64 math,
+ * fp32 out).  Replaces kappa_inv_brems (src/core/propagator.py:116-164) when
+ * called on its own; Te/Z have 1 or n elements; coulomb_log as in slt_grid_pack.
+ */
+int slt_kappa(const float* ne, int64_t n, const float* te, int64_t n_te, const float* zz,
+              int64_t n_z, double omega, double e_charge, double epsilon_0,
+              double speed_of_light, double coulomb_log, float* out, slt_stream_t stream);
+
+/*
+ * Save schedule: the integration runs from t = 0 through n_seg segments;
+ * segment s has seg_nsub[s] RK4 steps of length seg_h[s] seconds and ends at
+ * save point s+1 (save point 0 is the initial state).
+ */
+typedef struct slt_trace_args {
+    int64_t n_rays;
+    const float* s0;        /* device 6 x n_rays (rows x,y,z,vx,vy,vz), row stride n_rays */
+    const int32_t* order;   /* device processing order (a permutation of 0..n-1) or NULL  */
+    int32_t n_seg;          /* number of save intervals, >= 1                              */
+    int32_t save_every;     /* write save points to save_state/save_tau: 1 = all, 0 = last only */
+    const int32_t* seg_nsub;/* device [n_seg]                                              */
+    const float* seg_h;     /* device [n_seg], seconds                                     */
+    float* save_state;      /* device (n_seg+1) x 6 x n_rays (or 1 x 6 x n_rays if save_every == 0) */
+    double* save_tau;       /* device, same leading shape as save_state, 1 row; or NULL    */
+    double* path;           /* device [n_rays] optical path length [m] or NULL             */
+    uint32_t* steps;        /* device [n_rays] in-box RK4 steps per ray or NULL            */
+    unsigned long long* total_steps; /* device [1], incremented by all in-box steps, or NULL */
+} slt_trace_args;
+
+/*
+ * k2: fixed-step RK4 ray integrator (one ray per lane).  Replaces the vmapped
+ * diffrax solve of src/core/propagator.py:726-793 and :1162-1198 with the RHS
+ * dsdt (:168-243) and trilinear interpolation (src/core/interpolator.py:15-106):
+ *   dr/dt = v,  dv/dt = interp(grad(coeff ne))(r),  dtau/dt = interp(kappa)(r)
+ * with the interpolant 0 outside the closed grid box.  Positions and velocities
+ * are carried as compensated fp32 pairs; tau and path accumulate in fp64.
+ */
+int slt_trace(const void* packed, const slt_grid_desc* g, const slt_trace_args* a,
+              slt_stream_t stream);
+
+/*
+ * Jones-vector epilogue: state (6 x n, row stride ld) -> rf (4 x n, row stride n).
+ * Replaces ray_to_Jonesvector(..., keep_current_plane=True)
+ * (src/shared/propagation.py:4-90): rows [pos1, atan(v1/vp), pos2, atan(v2/vp)].
+ * If amp != NULL and tau != NULL, also writes amp[i] = exp(-tau[i])
+ * (src/core/propagator.py:324-326).
+ */
+int slt_jones(const float* state, int64_t ld, int64_t n, int32_t probing_axis, float* rf,
+              const double* tau, double* amp, slt_stream_t stream);
+
+/*
+ * k3: detector histogram.  Replaces numpy.histogram2d inside
+ * Diagnostic.histogram (src/processing/diagnostics.py:243-276) including its
+ * H.T transpose: counts/wsum are (nyb, nxb) row-major.  Bin b holds
+ * xedges[b] <= x < xedges[b+1]; x == xedges[nxb] goes to the last bin;
+ * anything else outside, and NaN, is dropped (numpy's rule).  Values are
+ * compared in fp64 against the fp64 edge arrays (device, nxb+1 / nyb+1 entries).
+ * x, y are device arrays of fp32 (is_f64 = 0) or fp64 (is_f64 = 1) with the
+ * given element strides -- e.g. rows 0 and 2 of rf.
+ * counts (uint32) and/or wsum (fp64, needs weights) are accumulated into, so
+ * the caller zeroes them first.  mask: optional device uint8 per ray (0 = skip).
+ */
+int slt_histogram2d(const void* x, int64_t x_stride, const void* y, int64_t y_stride,
+                    int32_t is_f64, int64_t n, const uint8_t* mask, const double* xedges,
+                    int32_t nxb, const double* yedges, int32_t nyb, const double* weights,
+                    uint32_t* counts, double* wsum, slt_stream_t stream);
+
+/*
+ * Morton (Z-order) keys of the rays' start positions in the plane transverse
+ * to the probing axis, quantised to 16 bits per axis over [lo, hi] of the grid
+ * (clamped).  Sorting rays by these keys keeps each wavefront's corner gathers
+ * on a few cache lines.  No reference counterpart (the reference vmaps rays in
+ * input order, src/core/propagator.py:792).
+ */
+int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int64_t* keys,
+                    slt_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLOTH_HIP_H */

@@ -1,0 +1,130 @@
+"""ctypes binding of libsloth_hip.so (the C ABI declared in include/sloth_hip.h).
+
+The product path has no CPU fallback: if the library is missing, or no HIP
+device is visible when a kernel is called, these helpers raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libsloth_hip.so")
+
+SLT_OK = 0
+SLT_ERR_INVALID = -1
+SLT_ERR_HIP = -2
+SLT_ERR_UNSUPPORTED = -3
+
+# Every symbol include/sloth_hip.h declares (tests check the library exports them).
+EXPORTED = (
+    "slt_version", "slt_last_error", "slt_grid_desc_init", "slt_grid_bytes", "slt_grid_pack",
+    "slt_kappa", "slt_trace", "slt_jones", "slt_histogram2d", "slt_morton_keys",
+)
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32 * 3),
+        ("nchan", ctypes.c_int32),
+        ("stride", ctypes.c_int64 * 3),
+        ("lo", ctypes.c_float * 3),
+        ("hi", ctypes.c_float * 3),
+        ("inv_d", ctypes.c_float * 3),
+        ("probing_axis", ctypes.c_int32),
+    ]
+
+
+class TraceArgs(ctypes.Structure):
+    _fields_ = [
+        ("n_rays", ctypes.c_int64),
+        ("s0", ctypes.c_void_p),
+        ("order", ctypes.c_void_p),
+        ("n_seg", ctypes.c_int32),
+        ("save_every", ctypes.c_int32),
+        ("seg_nsub", ctypes.c_void_p),
+        ("seg_h", ctypes.c_void_p),
+        ("save_state", ctypes.c_void_p),
+        ("save_tau", ctypes.c_void_p),
+        ("path", ctypes.c_void_p),
+        ("steps", ctypes.c_void_p),
+        ("total_steps", ctypes.c_void_p),
+    ]
+
+
+_lib = None
+
+
+class SlothError(RuntimeError):
+    """A HIP runtime failure reported by libsloth_hip."""
+
+
+def lib():
+    """Load libsloth_hip.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python -m sloth_amd.build, or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    L.slt_version.restype = ctypes.c_int
+    L.slt_last_error.restype = ctypes.c_char_p
+    L.slt_grid_desc_init.argtypes = [vp, vp, vp, i32, i32, i32, i32, i32, ctypes.POINTER(GridDesc)]
+    L.slt_grid_bytes.argtypes = [ctypes.POINTER(GridDesc)]
+    L.slt_grid_bytes.restype = i64
+    L.slt_grid_pack.argtypes = [vp, vp, vp, vp, dbl, vp, i64, vp, i64, dbl, dbl, dbl, dbl, dbl,
+                                ctypes.POINTER(GridDesc), vp, vp, vp, vp, vp, vp]
+    L.slt_kappa.argtypes = [vp, i64, vp, i64, vp, i64, dbl, dbl, dbl, dbl, dbl, vp, vp]
+    L.slt_trace.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(TraceArgs), vp]
+    L.slt_jones.argtypes = [vp, i64, i64, i32, vp, vp, vp, vp]
+    L.slt_histogram2d.argtypes = [vp, i64, vp, i64, i32, i64, vp, vp, i32, vp, i32, vp, vp, vp, vp]
+    L.slt_morton_keys.argtypes = [vp, i64, ctypes.POINTER(GridDesc), vp, vp]
+    for name in EXPORTED:
+        getattr(L, name).restype = getattr(L, name).restype or ctypes.c_int
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    msg = lib().slt_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str = "") -> None:
+    """Raise the Python exception matching a libsloth status code."""
+    if rc == SLT_OK:
+        return
+    msg = last_error() or what
+    if rc == SLT_ERR_INVALID:
+        raise ValueError(msg)
+    if rc == SLT_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise SlothError(msg)
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None for None)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(stream=None) -> int | None:
+    """hipStream_t of a torch stream (current stream by default)."""
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream or None
+
+
+def require_device():
+    """The HIP path is the only path: fail loudly without a visible GPU."""
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("sloth_amd needs a HIP device (MI355X); none is visible")
+    lib()
+    return torch.device("cuda", torch.cuda.current_device())

@@ -1,0 +1,260 @@
+"""Device-resident packed grids and RK4 ray traces (the layer under ``propagator``).
+
+``PackedGrid`` runs k1 (gradient stencil + inverse-bremsstrahlung rate + 16-byte
+voxel pack) once per density field and keeps the voxels in HBM; ``trace`` runs
+k2 (the RK4 integrator) over a (6, N) ray tensor with a save schedule.  Both go
+through the C ABI of ``include/sloth_hip.h``; PyTorch only provides device
+memory and the stream.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import _native as N
+from ..constants import ALPHA_CRIT, C_LIGHT, E_CHARGE, EPSILON_0
+
+AXES = {"x": 0, "y": 1, "z": 2}
+
+
+def axis_index(probing_direction: str) -> int:
+    if probing_direction not in AXES:
+        raise AssertionError("Invalid entry for probing_direction! Use x, y or z.")
+    return AXES[probing_direction]
+
+
+def grad_coeff(omega: float) -> float:
+    """-0.5 c^2 / (3.14207787e-4 omega^2) in float64 (src/core/propagator.py:88)."""
+    return float(np.float64(-0.5) * float(C_LIGHT) ** 2 / (ALPHA_CRIT * float(omega) ** 2))
+
+
+def critical_density(omega: float) -> float:
+    """n_cr = 3.14207787e-4 omega^2 [m^-3] (src/core/propagator.py:73)."""
+    return ALPHA_CRIT * float(omega) ** 2
+
+
+def host_f32(a) -> np.ndarray:
+    if isinstance(a, torch.Tensor):
+        return a.detach().to("cpu", torch.float32).contiguous().numpy()
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def device_f32(a, device) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        return a.detach().to(device=device, dtype=torch.float32).contiguous()
+    return torch.from_numpy(host_f32(a)).to(device)
+
+
+def _cl(coulomb_log) -> float:
+    return float(coulomb_log) if coulomb_log is not None and coulomb_log > 0 else 0.0
+
+
+class PackedGrid:
+    """A density field packed for tracing: 16 B per grid point, resident in HBM.
+
+    Voxel (i, j, k) holds (d/dx, d/dy, d/dz) of ``coeff * ne`` and, when
+    ``inv_brems``, the inverse-bremsstrahlung rate kappa [1/s].  The probing
+    axis is the fastest-varying axis of the layout.
+    """
+
+    def __init__(self, ne, x, y, z, omega, *, probing_direction="z", inv_brems=False,
+                 Te=None, Z=None, coulomb_log=None, device=None, stream=None):
+        dev = device if device is not None else N.require_device()
+        L = N.lib()
+        self.omega = float(omega)
+        self.axis = axis_index(probing_direction)
+        self.probing_direction = probing_direction
+        self.inv_brems = bool(inv_brems)
+        self.coords = [host_f32(c).reshape(-1) for c in (x, y, z)]
+        nx, ny, nz = (c.size for c in self.coords)
+        ne_d = device_f32(ne, dev)
+        if tuple(ne_d.shape) != (nx, ny, nz):
+            raise AssertionError(
+                f"ne has shape {tuple(ne_d.shape)} but the coordinates give {(nx, ny, nz)}")
+        desc = N.GridDesc()
+        xs, ys, zs = self.coords
+        N.check(L.slt_grid_desc_init(xs.ctypes.data, ys.ctypes.data, zs.ctypes.data, nx, ny, nz,
+                                     self.axis, 1 if self.inv_brems else 0, ctypes.byref(desc)))
+        self.desc = desc
+        self.shape = (nx, ny, nz)
+        self.nbytes = int(L.slt_grid_bytes(ctypes.byref(desc)))
+        self.voxels = torch.empty(self.nbytes // 4, dtype=torch.float32, device=dev)
+        cd = [torch.from_numpy(c).to(dev) for c in self.coords]
+        te_d = z_d = None
+        if self.inv_brems:
+            if Te is None or Z is None:
+                raise AssertionError("Te and Z must both be provided when inv_brems=True.")
+            te_d = device_f32(Te, dev).reshape(-1)
+            z_d = device_f32(Z, dev).reshape(-1)
+        s = N.stream_handle(stream)
+        N.check(L.slt_grid_pack(
+            ne_d.data_ptr(), cd[0].data_ptr(), cd[1].data_ptr(), cd[2].data_ptr(),
+            grad_coeff(omega), N.ptr(te_d), te_d.numel() if te_d is not None else 0,
+            N.ptr(z_d), z_d.numel() if z_d is not None else 0, self.omega, E_CHARGE, EPSILON_0,
+            C_LIGHT, _cl(coulomb_log), ctypes.byref(desc), self.voxels.data_ptr(),
+            None, None, None, None, s))
+        self.device = dev
+        self.spacing = np.array([1.0 / float(desc.inv_d[a]) for a in range(3)])
+
+    @property
+    def min_spacing(self) -> float:
+        return float(self.spacing.min())
+
+    def voxel_view(self) -> torch.Tensor:
+        """(nx, ny, nz, 4) view in (x, y, z) index order (strided, no copy)."""
+        st = [int(self.desc.stride[a]) * 4 for a in range(3)]
+        return torch.as_strided(self.voxels, (*self.shape, 4), (*st, 1))
+
+
+def grid_fields(ne, x, y, z, omega, *, Te=None, Z=None, coulomb_log=None, want_kappa=False,
+                device=None):
+    """Planar fp32 (dndx, dndy, dndz[, kappa]) grids from k1 (no packing)."""
+    dev = device if device is not None else N.require_device()
+    L = N.lib()
+    coords = [host_f32(c).reshape(-1) for c in (x, y, z)]
+    nx, ny, nz = (c.size for c in coords)
+    ne_d = device_f32(ne, dev)
+    if tuple(ne_d.shape) != (nx, ny, nz):
+        raise AssertionError(
+            f"ne has shape {tuple(ne_d.shape)} but the coordinates give {(nx, ny, nz)}")
+    desc = N.GridDesc()
+    N.check(L.slt_grid_desc_init(coords[0].ctypes.data, coords[1].ctypes.data,
+                                 coords[2].ctypes.data, nx, ny, nz, 2, 1 if want_kappa else 0,
+                                 ctypes.byref(desc)))
+    outs = [torch.empty((nx, ny, nz), dtype=torch.float32, device=dev)
+            for _ in range(4 if want_kappa else 3)]
+    cd = [torch.from_numpy(c).to(dev) for c in coords]
+    te_d = z_d = None
+    if want_kappa:
+        te_d = device_f32(Te, dev).reshape(-1)
+        z_d = device_f32(Z, dev).reshape(-1)
+    N.check(L.slt_grid_pack(
+        ne_d.data_ptr(), cd[0].data_ptr(), cd[1].data_ptr(), cd[2].data_ptr(), grad_coeff(omega),
+        N.ptr(te_d), te_d.numel() if te_d is not None else 0, N.ptr(z_d),
+        z_d.numel() if z_d is not None else 0, float(omega), E_CHARGE, EPSILON_0, C_LIGHT,
+        _cl(coulomb_log), ctypes.byref(desc), None, outs[0].data_ptr(), outs[1].data_ptr(),
+        outs[2].data_ptr(), outs[3].data_ptr() if want_kappa else None, N.stream_handle()))
+    return tuple(outs)
+
+
+def kappa_field(ne, Te, Z, omega, *, coulomb_log=None, device=None) -> torch.Tensor:
+    """Element-wise inverse-bremsstrahlung rate [1/s] (fp64 math, fp32 result)."""
+    dev = device if device is not None else N.require_device()
+    ne_d = device_f32(ne, dev)
+    shape = ne_d.shape
+    ne1 = ne_d.reshape(-1)
+    te_d = device_f32(Te, dev).reshape(-1)
+    z_d = device_f32(Z, dev).reshape(-1)
+    if te_d.numel() not in (1, ne1.numel()) or z_d.numel() not in (1, ne1.numel()):
+        if ne1.numel() == 1:
+            # scalar density against Te/Z grids: broadcast the density
+            ne1 = ne1.expand(max(te_d.numel(), z_d.numel())).contiguous()
+            shape = (te_d if te_d.numel() > 1 else z_d).shape
+        else:
+            raise ValueError("Te and Z must be scalars or have the shape of ne")
+    out = torch.empty(ne1.numel(), dtype=torch.float32, device=dev)
+    N.check(N.lib().slt_kappa(ne1.data_ptr(), ne1.numel(), te_d.data_ptr(), te_d.numel(),
+                              z_d.data_ptr(), z_d.numel(), float(omega), E_CHARGE, EPSILON_0,
+                              C_LIGHT, _cl(coulomb_log), out.data_ptr(), N.stream_handle()))
+    return out.reshape(shape)
+
+
+def schedule(save_times, h_nominal: float):
+    """Split [t_0, t_{S-1}] at the save times into RK4 steps of at most h_nominal.
+
+    Segment s (between save s and s+1) gets nsub[s] = ceil(dt_s / h_nominal)
+    equal steps of h[s] = dt_s / nsub[s], so every save time is hit exactly.
+    """
+    t = np.asarray(save_times, dtype=np.float64).reshape(-1)
+    if t.size < 2:
+        raise ValueError("need at least two save times")
+    dt = np.diff(t)
+    if np.any(dt < 0):
+        raise ValueError("save times must be non-decreasing")
+    nsub = np.ceil(dt / h_nominal - 1e-9).astype(np.int64)
+    nsub = np.maximum(nsub, 0)
+    h = np.where(nsub > 0, dt / np.maximum(nsub, 1), 0.0)
+    return nsub.astype(np.int32), h
+
+
+def nominal_step(grid: PackedGrid, step_fraction: float) -> float:
+    """RK4 time step: step_fraction of the smallest cell edge, at the speed of light."""
+    if not step_fraction > 0:
+        raise ValueError("step_fraction must be positive")
+    return step_fraction * grid.min_spacing / C_LIGHT
+
+
+@dataclass
+class TraceResult:
+    states: torch.Tensor           # (S, 6, N) float32: x, y, z, vx, vy, vz at each save
+    tau: torch.Tensor | None       # (S, N) float64 optical depth (inv_brems only)
+    path: torch.Tensor | None      # (N,) float64 path length [m]
+    steps: torch.Tensor            # (N,) int32 in-box RK4 steps per ray
+    total_steps: torch.Tensor      # (1,) int64 sum of steps
+    order: torch.Tensor | None     # processing order used (Morton sort) or None
+    nsub: np.ndarray
+    h: np.ndarray
+
+
+def ray_order(grid: PackedGrid, s0: torch.Tensor, stream=None) -> torch.Tensor:
+    """Z-order of the rays' transverse start positions (int32 permutation)."""
+    n = s0.shape[1]
+    keys = torch.empty(n, dtype=torch.int64, device=s0.device)
+    N.check(N.lib().slt_morton_keys(s0.data_ptr(), n, ctypes.byref(grid.desc), keys.data_ptr(),
+                                    N.stream_handle(stream)))
+    return torch.sort(keys, stable=True).indices.to(torch.int32)
+
+
+def as_rays(s0, device) -> torch.Tensor:
+    """(6, N) float32 contiguous device tensor from any array-like."""
+    t = device_f32(s0, device)
+    if t.dim() != 2 or t.shape[0] != 6:
+        raise AssertionError("Expected a (6, N) matrix of pre-created rays.")
+    return t
+
+
+def trace(grid: PackedGrid, s0, save_times, *, step_fraction: float = 0.5,
+          save_all: bool = True, want_path: bool = True, sort: bool = True,
+          order: torch.Tensor | None = None, stream=None) -> TraceResult:
+    """Integrate rays through ``grid`` with fixed-step RK4, saving at ``save_times``.
+
+    ``save_times`` are real times [s] starting at 0 (the initial state).
+    """
+    rays = as_rays(s0, grid.device)
+    n = rays.shape[1]
+    nsub, h = schedule(save_times, nominal_step(grid, step_fraction))
+    if int(nsub.astype(np.int64).sum()) >= 2 ** 31:
+        raise NotImplementedError("more than 2^31 RK4 steps per ray")
+    dev = grid.device
+    nseg = nsub.size
+    S = nseg + 1 if save_all else 1
+    states = torch.empty((S, 6, n), dtype=torch.float32, device=dev)
+    tau = torch.empty((S, n), dtype=torch.float64, device=dev) if grid.inv_brems else None
+    path = torch.empty(n, dtype=torch.float64, device=dev) if want_path else None
+    steps = torch.empty(n, dtype=torch.int32, device=dev)
+    total = torch.zeros(1, dtype=torch.int64, device=dev)
+    if n == 0:
+        return TraceResult(states, tau, path, steps, total, None, nsub, h)
+    if order is None and sort:
+        order = ray_order(grid, rays, stream)
+    seg_nsub = torch.from_numpy(nsub).to(dev)
+    seg_h = torch.from_numpy(h.astype(np.float32)).to(dev)
+    a = N.TraceArgs()
+    a.n_rays = n
+    a.s0 = rays.data_ptr()
+    a.order = N.ptr(order)
+    a.n_seg = nseg
+    a.save_every = 1 if save_all else 0
+    a.seg_nsub = seg_nsub.data_ptr()
+    a.seg_h = seg_h.data_ptr()
+    a.save_state = states.data_ptr()
+    a.save_tau = N.ptr(tau)
+    a.path = N.ptr(path)
+    a.steps = steps.data_ptr()
+    a.total_steps = total.data_ptr()
+    N.check(N.lib().slt_trace(grid.voxels.data_ptr(), ctypes.byref(grid.desc), ctypes.byref(a),
+                              N.stream_handle(stream)))
+    return TraceResult(states, tau, path, steps, total, order, nsub, h)

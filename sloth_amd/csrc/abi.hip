@@ -1,0 +1,145 @@
+// Error handling, host-side grid descriptors and ray ordering for libsloth_hip.so.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "sloth_internal.h"
+
+namespace slt {
+
+static thread_local char g_err[512] = {0};
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+void clear_error() { g_err[0] = 0; }
+
+int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("%s: %s", what, hipGetErrorString(e));
+        return SLT_ERR_HIP;
+    }
+    return SLT_OK;
+}
+
+// Spreads the low 16 bits of v to the even bit positions of a 32-bit word.
+__device__ inline uint32_t spread16(uint32_t v) {
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__device__ inline uint32_t quantise16(float p, float lo, float hi) {
+    float u = (p - lo) / (hi - lo);
+    u = fminf(fmaxf(u, 0.0f), 1.0f);  // NaN -> 0
+    return (uint32_t)(u * 65535.0f);
+}
+
+__global__ void __launch_bounds__(256) morton_kernel(const float* __restrict__ s0, int64_t n,
+                                                     int ax_a, int ax_b, float lo_a, float hi_a,
+                                                     float lo_b, float hi_b,
+                                                     int64_t* __restrict__ keys) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t qa = quantise16(s0[ax_a * n + i], lo_a, hi_a);
+    const uint32_t qb = quantise16(s0[ax_b * n + i], lo_b, hi_b);
+    keys[i] = (int64_t)(spread16(qa) | (spread16(qb) << 1));
+}
+
+}  // namespace slt
+
+using namespace slt;
+
+extern "C" {
+
+int slt_version(void) { return SLT_ABI_VERSION; }
+
+const char* slt_last_error(void) { return g_err; }
+
+int slt_grid_desc_init(const float* x, const float* y, const float* z, int32_t nx, int32_t ny,
+                       int32_t nz, int32_t probing_axis, int32_t with_kappa,
+                       slt_grid_desc* out) {
+    clear_error();
+    if (!x || !y || !z || !out) {
+        set_error("slt_grid_desc_init: null pointer");
+        return SLT_ERR_INVALID;
+    }
+    if (probing_axis < 0 || probing_axis > 2) {
+        set_error("slt_grid_desc_init: probing_axis must be 0, 1 or 2 (got %d)", probing_axis);
+        return SLT_ERR_INVALID;
+    }
+    const float* c[3] = {x, y, z};
+    const int32_t n[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) {
+        if (n[a] < 2) {
+            set_error("slt_grid_desc_init: axis %d has %d points (need >= 2)", a, n[a]);
+            return SLT_ERR_INVALID;
+        }
+        const double lo = c[a][0], hi = c[a][n[a] - 1];
+        if (!(hi > lo) || !isfinite(lo) || !isfinite(hi)) {
+            set_error("slt_grid_desc_init: axis %d coordinates must be finite and ascending", a);
+            return SLT_ERR_INVALID;
+        }
+        const double d = (hi - lo) / (n[a] - 1);
+        const double amax = fabs(lo) > fabs(hi) ? fabs(lo) : fabs(hi);
+        const double ulp = amax * 1.1920928955078125e-07;
+        const double tol = fmax(1e-4 * d, 4.0 * ulp);
+        for (int i = 0; i < n[a]; ++i) {
+            const double dev = fabs((double)c[a][i] - (lo + i * d));
+            if (!(dev <= tol)) {
+                set_error("slt_grid_desc_init: axis %d is not uniformly spaced "
+                          "(point %d deviates by %.3g cells); only linspace grids are supported",
+                          a, i, dev / d);
+                return SLT_ERR_UNSUPPORTED;
+            }
+        }
+        out->n[a] = n[a];
+        out->lo[a] = c[a][0];
+        out->hi[a] = c[a][n[a] - 1];
+        out->inv_d[a] = (float)((n[a] - 1) / ((double)out->hi[a] - (double)out->lo[a]));
+    }
+    // Probing axis fastest, the other two in x, y, z order (slowest first).
+    int others[2], k = 0;
+    for (int a = 0; a < 3; ++a)
+        if (a != probing_axis) others[k++] = a;
+    out->stride[probing_axis] = 1;
+    out->stride[others[1]] = n[probing_axis];
+    out->stride[others[0]] = (int64_t)n[probing_axis] * n[others[1]];
+    out->nchan = with_kappa ? 4 : 3;
+    out->probing_axis = probing_axis;
+    return SLT_OK;
+}
+
+int64_t slt_grid_bytes(const slt_grid_desc* g) {
+    if (!g) return 0;
+    return (int64_t)g->n[0] * g->n[1] * g->n[2] * 16;
+}
+
+int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int64_t* keys,
+                    slt_stream_t stream) {
+    clear_error();
+    if (n < 0 || !g || (n > 0 && (!s0 || !keys))) {
+        set_error("slt_morton_keys: invalid arguments");
+        return SLT_ERR_INVALID;
+    }
+    if (n == 0) return SLT_OK;
+    int ax[2], k = 0;
+    for (int a = 0; a < 3; ++a)
+        if (a != g->probing_axis) ax[k++] = a;
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(morton_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       (hipStream_t)stream, s0, n, ax[0], ax[1], g->lo[ax[0]], g->hi[ax[0]],
+                       g->lo[ax[1]], g->hi[ax[1]], keys);
+    return check_launch("slt_morton_keys");
+}
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+// k1: refraction-gradient stencil, inverse-bremsstrahlung rate and 16-byte
+// voxel packing.  One thread per grid point; fp64 arithmetic, one rounding to
+// fp32 at the store.  HBM-bound: reads 7 neighbouring fp32 densities (mostly
+// L2 hits) and writes one 16-byte voxel per point.
+#include <math.h>
+
+#include "sloth_internal.h"
+
+namespace slt {
+
+struct PackArgs {
+    const float* __restrict__ ne;
+    const float* __restrict__ c0;
+    const float* __restrict__ c1;
+    const float* __restrict__ c2;
+    double coeff;
+    const float* __restrict__ te;
+    int64_t n_te;
+    const float* __restrict__ zz;
+    int64_t n_z;
+    double omega, e_charge, epsilon_0, c_light, coulomb_log;
+    int n0, n1, n2;
+    int64_t s0, s1, s2;  // packed strides
+    float4* __restrict__ packed;
+    float* __restrict__ pdx;
+    float* __restrict__ pdy;
+    float* __restrict__ pdz;
+    float* __restrict__ pk;
+};
+
+// numpy/jnp.gradient along one axis with a coordinate array (edge_order=1):
+// src/core/propagator.py:90-92.
+__device__ inline double grad_axis(const float* __restrict__ ne, const float* __restrict__ c,
+                                   int64_t lin, int64_t stride, int i, int n, double coeff) {
+    const double f0 = (double)ne[lin] * coeff;
+    if (i == 0) {
+        const double f1 = (double)ne[lin + stride] * coeff;
+        return (f1 - f0) / ((double)c[1] - (double)c[0]);
+    }
+    if (i == n - 1) {
+        const double fm = (double)ne[lin - stride] * coeff;
+        return (f0 - fm) / ((double)c[n - 1] - (double)c[n - 2]);
+    }
+    const double fm = (double)ne[lin - stride] * coeff;
+    const double fp = (double)ne[lin + stride] * coeff;
+    const double dx1 = (double)c[i] - (double)c[i - 1];
+    const double dx2 = (double)c[i + 1] - (double)c[i];
+    const double a = -dx2 / (dx1 * (dx1 + dx2));
+    const double b = (dx2 - dx1) / (dx1 * dx2);
+    const double cc = dx1 / (dx2 * (dx1 + dx2));
+    return a * fm + b * f0 + cc * fp;
+}
+
+// NRL inverse-bremsstrahlung amplitude absorption rate [1/s]:
+// src/core/propagator.py:147-164.
+__device__ inline double kappa_ib(double ne, double te, double z, const PackArgs& A) {
+    const double pi = 3.14159265358979311600;
+    const double ne_cc = ne * 1e-6;
+    const double v_the = 4.19e5 * sqrt(te);
+    const double o_pe = 5.64e4 * sqrt(ne_cc);
+    const double o_max = o_pe > A.omega ? o_pe : A.omega;
+    const double b_min = z * A.e_charge / (4.0 * pi * A.epsilon_0 * te);
+    double cl = log(v_the / (o_max * b_min));
+    if (!(cl >= 2.0)) cl = 2.0;
+    if (A.coulomb_log > 0.0) cl = A.coulomb_log;
+    const double r = ne_cc / A.omega;
+    return 3.1e-5 * z * A.c_light * (r * r) * cl * pow(te, -1.5);
+}
+
+__global__ void __launch_bounds__(256) pack_kernel(PackArgs A) {
+    const int64_t total = (int64_t)A.n0 * A.n1 * A.n2;
+    const int64_t sy = A.n2, sx = (int64_t)A.n1 * A.n2;
+    for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < total;
+         lin += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(lin / sx);
+        const int64_t rem = lin - (int64_t)i * sx;
+        const int j = (int)(rem / sy);
+        const int k = (int)(rem - (int64_t)j * sy);
+        const double gx = grad_axis(A.ne, A.c0, lin, sx, i, A.n0, A.coeff);
+        const double gy = grad_axis(A.ne, A.c1, lin, sy, j, A.n1, A.coeff);
+        const double gz = grad_axis(A.ne, A.c2, lin, 1, k, A.n2, A.coeff);
+        double kap = 0.0;
+        if (A.te) {
+            const double te = (double)A.te[A.n_te == 1 ? 0 : lin];
+            const double z = (double)A.zz[A.n_z == 1 ? 0 : lin];
+            kap = kappa_ib((double)A.ne[lin], te, z, A);
+        }
+        const float4 v = make_float4((float)gx, (float)gy, (float)gz, (float)kap);
+        if (A.packed) A.packed[i * A.s0 + j * A.s1 + k * A.s2] = v;
+        if (A.pdx) A.pdx[lin] = v.x;
+        if (A.pdy) A.pdy[lin] = v.y;
+        if (A.pdz) A.pdz[lin] = v.z;
+        if (A.pk) A.pk[lin] = v.w;
+    }
+}
+
+__global__ void __launch_bounds__(256) kappa_kernel(PackArgs A, int64_t n, float* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double te = (double)A.te[A.n_te == 1 ? 0 : i];
+        const double z = (double)A.zz[A.n_z == 1 ? 0 : i];
+        out[i] = (float)kappa_ib((double)A.ne[i], te, z, A);
+    }
+}
+
+}  // namespace slt
+
+using namespace slt;
+
+extern "C" int slt_kappa(const float* ne, int64_t n, const float* te, int64_t n_te,
+                         const float* zz, int64_t n_z, double omega, double e_charge,
+                         double epsilon_0, double speed_of_light, double coulomb_log,
+                         float* out, slt_stream_t stream) {
+    clear_error();
+    if (n < 0 || (n > 0 && (!ne || !te || !zz || !out)) || (n_te != 1 && n_te != n) ||
+        (n_z != 1 && n_z != n)) {
+        set_error("slt_kappa: invalid arguments (Te/Z must have 1 or n elements)");
+        return SLT_ERR_INVALID;
+    }
+    if (n == 0) return SLT_OK;
+    PackArgs A = {};
+    A.ne = ne;
+    A.te = te; A.n_te = n_te;
+    A.zz = zz; A.n_z = n_z;
+    A.omega = omega; A.e_charge = e_charge; A.epsilon_0 = epsilon_0; A.c_light = speed_of_light;
+    A.coulomb_log = coulomb_log;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(kappa_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, A,
+                       n, out);
+    return check_launch("slt_kappa");
+}
+
+extern "C" int slt_grid_pack(const float* ne, const float* cx, const float* cy, const float* cz,
+                             double grad_coeff, const float* te, int64_t n_te, const float* zz,
+                             int64_t n_z, double omega, double e_charge, double epsilon_0,
+                             double speed_of_light, double coulomb_log, const slt_grid_desc* g,
+                             void* packed, float* planar_dx, float* planar_dy, float* planar_dz,
+                             float* planar_kappa, slt_stream_t stream) {
+    clear_error();
+    if (!ne || !cx || !cy || !cz || !g) {
+        set_error("slt_grid_pack: null input pointer");
+        return SLT_ERR_INVALID;
+    }
+    const int64_t total = (int64_t)g->n[0] * g->n[1] * g->n[2];
+    if (g->n[0] < 2 || g->n[1] < 2 || g->n[2] < 2) {
+        set_error("slt_grid_pack: every axis needs >= 2 points");
+        return SLT_ERR_INVALID;
+    }
+    const bool want_kappa = te != nullptr || zz != nullptr;
+    if (want_kappa) {
+        if (!te || !zz) {
+            set_error("slt_grid_pack: Te and Z must both be given for kappa");
+            return SLT_ERR_INVALID;
+        }
+        if ((n_te != 1 && n_te != total) || (n_z != 1 && n_z != total)) {
+            set_error("slt_grid_pack: Te/Z must have 1 or nx*ny*nz elements (got %lld, %lld)",
+                      (long long)n_te, (long long)n_z);
+            return SLT_ERR_INVALID;
+        }
+    }
+    if (!packed && !planar_dx && !planar_dy && !planar_dz && !planar_kappa) {
+        set_error("slt_grid_pack: no output requested");
+        return SLT_ERR_INVALID;
+    }
+    PackArgs A;
+    A.ne = ne; A.c0 = cx; A.c1 = cy; A.c2 = cz;
+    A.coeff = grad_coeff;
+    A.te = want_kappa ? te : nullptr; A.n_te = n_te;
+    A.zz = want_kappa ? zz : nullptr; A.n_z = n_z;
+    A.omega = omega; A.e_charge = e_charge; A.epsilon_0 = epsilon_0; A.c_light = speed_of_light;
+    A.coulomb_log = coulomb_log;
+    A.n0 = g->n[0]; A.n1 = g->n[1]; A.n2 = g->n[2];
+    A.s0 = g->stride[0]; A.s1 = g->stride[1]; A.s2 = g->stride[2];
+    A.packed = static_cast<float4*>(packed);
+    A.pdx = planar_dx; A.pdy = planar_dy; A.pdz = planar_dz; A.pk = planar_kappa;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, A);
+    return check_launch("slt_grid_pack");
+}

@@ -1,0 +1,100 @@
+"""Shared helpers for GPU-vs-oracle parity tests.
+
+Error measures (the tolerances these are checked against are stated in the
+tests; the north-star bar is 1e-5):
+  position   ||r_gpu - r_ref||_2 / max(||r_ref||_2, half-extent of the grid)
+  direction  ||v_gpu - v_ref||_2 / ||v_ref||_2
+  energy     |exp(-tau_gpu) - exp(-tau_ref)| / exp(-tau_ref)
+"""
+import numpy as np
+
+C = 299792458.0
+
+
+def gpu_trace(ne, coords, s0, save_times, *, lwl=532e-9, inv_brems=False, Te=None, Z=None,
+              coulomb_log=None, probing_direction="z", step_fraction=0.5, sort=True):
+    import torch
+
+    from sloth_amd.core import engine
+
+    omega = 2 * np.pi * C / lwl
+    grid = engine.PackedGrid(ne, *coords, omega, probing_direction=probing_direction,
+                             inv_brems=inv_brems, Te=Te, Z=Z, coulomb_log=coulomb_log)
+    res = engine.trace(grid, s0, save_times, step_fraction=step_fraction, save_all=True,
+                       want_path=True, sort=sort)
+    torch.cuda.synchronize()
+    out = {
+        "states": res.states.cpu().numpy().astype(np.float64),
+        "tau": res.tau.cpu().numpy() if res.tau is not None else None,
+        "path": res.path.cpu().numpy(),
+        "steps": res.steps.cpu().numpy().astype(np.int64),
+        "total": int(res.total_steps.item()),
+        "nsub": res.nsub,
+        "h": res.h,
+        "grid": grid,
+    }
+    return out
+
+
+def oracle_trace(orc, ne, coords, s0, nsub, h, *, lwl=532e-9, inv_brems=False, Te=None, Z=None,
+                 coulomb_log=None):
+    omega = 2 * np.pi * C / lwl
+    ne64 = np.asarray(ne, dtype=np.float32).astype(np.float64)
+    c64 = [np.asarray(c, dtype=np.float32).astype(np.float64) for c in coords]
+    grads = orc.gradients(ne64, *c64, omega)
+    kap = None
+    if inv_brems:
+        te = np.asarray(Te, dtype=np.float32).astype(np.float64)
+        zz = np.asarray(Z, dtype=np.float32).astype(np.float64)
+        kap = orc.kappa(ne64, te, zz, omega, coulomb_log=coulomb_log)
+    # the kernel steps with the fp32-rounded step lengths; give the oracle the same
+    h32 = np.asarray(h, dtype=np.float32).astype(np.float64)
+    s64 = np.asarray(s0, dtype=np.float32).astype(np.float64)
+    saves, steps = orc.trace(grads, kap, c64, s64, nsub, h32)
+    return {"saves": saves, "steps": steps, "grads": grads, "kappa": kap}
+
+
+def errors(gpu, ref, coords):
+    """Per-save, per-ray error arrays (position, direction, energy)."""
+    half = max(max(abs(float(c[0])), abs(float(c[-1]))) for c in coords)
+    g = gpu["states"]
+    r = ref["saves"]
+    dr = np.linalg.norm(g[:, 0:3] - r[:, 0:3], axis=1)
+    rn = np.maximum(np.linalg.norm(r[:, 0:3], axis=1), half)
+    dv = np.linalg.norm(g[:, 3:6] - r[:, 3:6], axis=1)
+    vn = np.linalg.norm(r[:, 3:6], axis=1)
+    out = {"pos": dr / rn, "dir": dv / vn}
+    if gpu["tau"] is not None:
+        eg = np.exp(-gpu["tau"])
+        er = np.exp(-r[:, 6])
+        out["energy"] = np.abs(eg - er) / er
+        out["tau_abs"] = np.abs(gpu["tau"] - r[:, 6])
+    out["path"] = np.abs(gpu["path"] - r[-1, 7]) / np.maximum(r[-1, 7], 1e-30)
+    return out
+
+
+def assert_parity(gpu, ref, coords, tol=1e-5, step_mismatch_frac=1e-3):
+    e = errors(gpu, ref, coords)
+    msg = {k: float(np.max(v)) for k, v in e.items()}
+    assert np.max(e["pos"]) <= tol, msg
+    assert np.max(e["dir"]) <= tol, msg
+    if "energy" in e:
+        assert np.max(e["energy"]) <= tol, msg
+    assert np.max(e["path"]) <= tol, msg
+    ds = np.abs(gpu["steps"] - ref["steps"])
+    assert ds.max() <= 1, ("step counts differ by more than one", ds.max())
+    assert np.mean(ds > 0) <= step_mismatch_frac, np.mean(ds > 0)
+    return msg
+
+
+def rays(x0, y0, z0, vx=0.0, vy=0.0):
+    x0 = np.atleast_1d(np.asarray(x0, dtype=np.float64))
+    n = x0.size
+    s0 = np.zeros((6, n))
+    s0[0] = x0
+    s0[1] = y0
+    s0[2] = z0
+    s0[3] = vx
+    s0[4] = vy
+    s0[5] = np.sqrt(C ** 2 - np.asarray(vx) ** 2 - np.asarray(vy) ** 2)
+    return s0.astype(np.float32)

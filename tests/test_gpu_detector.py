@@ -1,0 +1,116 @@
+"""Jones epilogue and detector histogram (k3) against numpy / the oracle."""
+import numpy as np
+import pytest
+
+from _parity import C, gpu_trace, oracle_trace
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _dev():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def _edge_heavy_sample(n, lo, hi, nb, rng, dtype):
+    e = np.linspace(lo, hi, nb + 1)
+    v = rng.uniform(lo - 0.1 * (hi - lo), hi + 0.1 * (hi - lo), n)
+    k = n // 5
+    v[:k] = e[rng.integers(0, nb + 1, k)]          # exactly on edges, incl. both ends
+    v[k:k + 10] = hi
+    v[k + 10:k + 20] = np.nan
+    return v.astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_histogram_bit_exact_vs_numpy(dtype):
+    import torch
+
+    from sloth_amd.processing.diagnostics import histogram2d_device
+
+    rng = np.random.default_rng(20)
+    n = 200_000
+    x = _edge_heavy_sample(n, -9.0, 9.0, 256, rng, dtype)
+    y = _edge_heavy_sample(n, -6.75, 6.75, 192, rng, dtype)
+    xe, ye = np.linspace(-9.0, 9.0, 257), np.linspace(-6.75, 6.75, 193)
+    w = rng.uniform(0, 1, n)
+    counts, wsum = histogram2d_device(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), xe,
+                                      ye, weights=torch.from_numpy(w))
+    keep = ~np.isnan(x) & ~np.isnan(y)
+    H, _, _ = np.histogram2d(x[keep], y[keep], bins=[256, 192], range=[[-9, 9], [-6.75, 6.75]])
+    Hw, _, _ = np.histogram2d(x[keep], y[keep], bins=[256, 192], range=[[-9, 9], [-6.75, 6.75]],
+                              weights=w[keep])
+    np.testing.assert_array_equal(counts.cpu().numpy(), H.T.astype(np.int64))
+    np.testing.assert_allclose(wsum.cpu().numpy(), Hw.T, rtol=1e-12, atol=1e-12)
+
+
+def test_diagnostic_histogram_matches_numpy_and_weights_contract():
+    from sloth_amd.processing.diagnostics import Diagnostic
+
+    rng = np.random.RandomState(42)
+    rf = np.zeros((4, 500))
+    rf[0] = (rng.rand(500) - 0.5) * 40
+    rf[2] = (rng.rand(500) - 0.5) * 40
+    rf[1] = rng.randn(500) * 1e-4
+    rf[3] = rng.randn(500) * 1e-4
+    d = Diagnostic(rf, L=100, R=500, Lx=50, Ly=50)
+    d.histogram(pix_x=20, pix_y=30)
+    H, xe, ye = np.histogram2d(rf[0], rf[2], bins=[20, 30], range=[[-25, 25], [-25, 25]])
+    np.testing.assert_array_equal(d.H, H.T)
+    np.testing.assert_array_equal(d.xedges, xe)
+    ones = Diagnostic(rf, weights=np.ones(500), L=100, R=500, Lx=50, Ly=50)
+    ones.histogram(pix_x=20, pix_y=30)
+    np.testing.assert_array_equal(ones.H, d.H)            # unit weights == counts
+    low = Diagnostic(rf, weights=np.full(500, 0.1), L=100, R=500, Lx=50, Ly=50)
+    low.histogram(pix_x=20, pix_y=30)
+    assert abs(low.H.sum() / d.H.sum() - 0.1) < 1e-12
+    # CCD mask on post-RTM (mm) coordinates reduces the count
+    ccd = Diagnostic(rf, ccd_shape_m=(13.5e-3, 18e-3), L=100, R=500, Lx=50, Ly=50)
+    ccd.rf = rf.copy()
+    ccd.histogram(pix_x=20, pix_y=20)
+    inside = (np.abs(rf[0]) <= 6.75) & (np.abs(rf[2]) <= 9.0)
+    H2, _, _ = np.histogram2d(rf[0][inside], rf[2][inside], bins=[20, 20], range=[[-25, 25]] * 2)
+    np.testing.assert_array_equal(ccd.H, H2.T)
+
+
+def test_traced_detector_image_matches_oracle_except_edge_rays(orc):
+    """256x256 detector image of traced rays: counts equal the oracle's except for rays whose
+    oracle position lies within the parity tolerance of a bin edge."""
+    import torch
+
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import sine_field
+    from sloth_amd.processing.diagnostics import histogram2d_device
+    from sloth_amd.shared.propagation import jones_device
+
+    L = 10e-3
+    ne, coords = sine_field(64, L)
+    s0 = Beam(20000, 2e-3, 0.0, 5e-3, beam_type="circular", seeded=True).s0
+    # solve() horizon; a horizon of exactly L/c would put every collimated ray's last
+    # stage point on the exit face, where rounding decides inside/outside
+    t = np.array([0.0, np.sqrt(8.0) * L / C])
+    gpu = gpu_trace(ne, coords, s0, t, lwl=1064e-9)
+    ref = oracle_trace(orc, ne, coords, s0, gpu["nsub"], gpu["h"], lwl=1064e-9)
+    st = torch.from_numpy(gpu["states"][-1].astype(np.float32)).cuda()
+    rf, _ = jones_device(st, "z")
+    rf_ref = orc.jones(ref["saves"][-1][:6], "z")
+    # angles [rad]: the 1e-5 direction bar
+    np.testing.assert_allclose(rf.cpu().numpy()[[1, 3]], rf_ref[[1, 3]], rtol=0, atol=1e-5)
+    xe = np.linspace(-4e-3, 4e-3, 257)
+    ye = np.linspace(-4e-3, 4e-3, 257)
+    counts, _ = histogram2d_device(rf[0], rf[2], xe, ye)
+    H_gpu = counts.cpu().numpy()
+    # numpy on the GPU's own positions: bit-exact
+    H_np, _, _ = np.histogram2d(rf[0].cpu().numpy(), rf[2].cpu().numpy(), bins=[xe, ye])
+    np.testing.assert_array_equal(H_gpu, H_np.T)
+    # oracle positions: only rays within tol of an edge may land elsewhere
+    H_ref, _, _ = np.histogram2d(rf_ref[0], rf_ref[2], bins=[xe, ye])
+    tol = 1e-5 * np.sqrt(8.0) * L
+    near = np.zeros(rf_ref.shape[1], bool)
+    for v, e in ((rf_ref[0], xe), (rf_ref[2], ye)):
+        near |= np.min(np.abs(v[:, None] - e[None, :]), axis=1) <= tol
+    diff = np.abs(H_gpu - H_ref.T).sum()
+    assert diff <= 2 * near.sum(), (diff, near.sum())

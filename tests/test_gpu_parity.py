@@ -1,0 +1,223 @@
+"""GPU (HIP) path vs the FP64 oracle on identical seeded inputs.
+
+Tolerance: 1e-5 on position (normwise, relative to max(|r|, grid half-extent)),
+direction (normwise relative) and transmitted energy exp(-tau) (relative),
+and on path length; in-box step counts may differ by one step for at most
+0.1 % of rays (a stage point within rounding of a box face).
+"""
+import numpy as np
+import pytest
+
+from _parity import C, assert_parity, errors, gpu_trace, oracle_trace, rays
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _dev():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from sloth_amd import _native as N
+
+    N.lib()
+
+
+def _sine(n=64, L=10e-3):
+    from sloth_amd.fields import sine_field
+
+    return sine_field(n, L)
+
+
+# ---------------------------------------------------------------------------
+# k1: pack
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("pd", ["x", "y", "z"])
+def test_pack_matches_oracle(orc, pd):
+    import torch
+
+    from sloth_amd.core import engine
+
+    rng = np.random.default_rng(10)
+    shape = (17, 19, 23)
+    ne = rng.uniform(1e23, 5e25, shape).astype(np.float32)
+    coords = [np.float32(np.linspace(-a, a, n)) for a, n in zip((1e-3, 2e-3, 3e-3), shape)]
+    te = rng.uniform(5, 200, shape).astype(np.float32)
+    z = np.float32(1.5)
+    omega = 2 * np.pi * C / 532e-9
+    g = engine.PackedGrid(ne, *coords, omega, probing_direction=pd, inv_brems=True, Te=te, Z=z)
+    vox = g.voxel_view().cpu().numpy().astype(np.float64)
+    c64 = [c.astype(np.float64) for c in coords]
+    grads = orc.gradients(ne.astype(np.float64), *c64, omega)
+    kap = orc.kappa(ne.astype(np.float64), te.astype(np.float64), 1.5, omega)
+    for ch, ref in enumerate([*grads, kap]):
+        scale = np.abs(ref).max()
+        np.testing.assert_allclose(vox[..., ch], ref, rtol=1e-6, atol=1e-6 * scale)
+    planar = engine.grid_fields(ne, *coords, omega, Te=te, Z=z, want_kappa=True)
+    for ch, p in enumerate(planar):
+        np.testing.assert_array_equal(p.cpu().numpy(), vox[..., ch].astype(np.float32))
+    torch.cuda.synchronize()
+
+
+def test_kappa_kernel_matches_oracle(orc):
+    from sloth_amd.core.propagator import kappa_inv_brems
+
+    omega = 2 * np.pi * C / 1064e-9
+    for te, z in [(100.0, 1.0), (10.0, 4.0), (5.0, 1.0)]:
+        got = float(kappa_inv_brems(np.float32(1e25), np.float32(te), z, omega))
+        want = orc.kappa(np.array([1e25]), te, z, omega)[0]
+        assert got == pytest.approx(want, rel=1e-6)
+    ne = np.random.default_rng(1).uniform(1e22, 1e26, (4, 5, 6)).astype(np.float32)
+    got = kappa_inv_brems(ne, 20.0, 1.0, omega, coulomb_log=10.0).cpu().numpy()
+    want = orc.kappa(ne.astype(np.float64), 20.0, 1.0, omega, coulomb_log=10.0)
+    np.testing.assert_allclose(got, want, rtol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# k2: trace
+# ---------------------------------------------------------------------------
+
+def test_trace_sine_field_1064nm(orc):
+    """Config-1 shape at reduced size: sine field, circular collimated beam, no absorption."""
+    from sloth_amd.core.beam import Beam
+
+    ne, coords = _sine(64, 10e-3)
+    s0 = Beam(4000, 2e-3, 0.0, 5e-3, beam_type="circular", seeded=True).s0
+    T = np.sqrt(8.0) * 10e-3 / C
+    t = np.linspace(0, T, 2)
+    gpu = gpu_trace(ne, coords, s0, t, lwl=1064e-9)
+    ref = oracle_trace(orc, ne, coords, s0, gpu["nsub"], gpu["h"], lwl=1064e-9)
+    msg = assert_parity(gpu, ref, coords)
+    print("sine parity", msg)
+    assert gpu["total"] == int(gpu["steps"].sum())
+    # every ray crosses the whole box: 10 mm at half a cell per step
+    assert np.median(gpu["steps"]) in (127, 128)
+
+
+def test_trace_quadratic_trough_closed_form(orc):
+    """Config-2 shape at reduced size; also against the closed-form oscillation."""
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import quadratic_trough
+
+    L = 12e-3
+    ne, coords = quadratic_trough(96, L)
+    s0 = Beam(4000, 4e-3, 0.0, L / 2, beam_type="square", seeded=True).s0
+    depth = 11e-3
+    t = np.linspace(0, depth, 12) / C
+    gpu = gpu_trace(ne, coords, s0, t, lwl=532e-9)
+    ref = oracle_trace(orc, ne, coords, s0, gpu["nsub"], gpu["h"], lwl=532e-9)
+    print("trough parity", assert_parity(gpu, ref, coords))
+    omega = 2 * np.pi * C / 532e-9
+    w = C * np.sqrt(5e24 / (3.14207787e-4 * omega ** 2)) / 3e-3
+    x0, y0 = s0[0].astype(np.float64), s0[1].astype(np.float64)
+    fin = gpu["states"][-1]
+    # trilinear interpolation of the exactly-linear gradient is exact inside the box
+    np.testing.assert_allclose(fin[0], x0 * np.cos(w * t[-1]), rtol=0, atol=2e-9)
+    np.testing.assert_allclose(fin[1], y0 * np.cos(w * t[-1]), rtol=0, atol=2e-9)
+
+
+def test_trace_turbulent_with_absorption(orc):
+    """Config-3 shape at reduced size: Kolmogorov field, absorption, both Coulomb logs."""
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import kolmogorov_field
+
+    L = 20e-3
+    ne, coords = kolmogorov_field(96, L, seed=3)
+    s0 = Beam(3000, 8e-3, 0.0, L / 2, beam_type="square", seeded=True).s0
+    t = np.array([0.0, 0.5, 1.0]) * L / C
+    for cl in (None, 10.0):
+        gpu = gpu_trace(ne, coords, s0, t, lwl=532e-9, inv_brems=True, Te=20.0, Z=1.0,
+                        coulomb_log=cl)
+        ref = oracle_trace(orc, ne, coords, s0, gpu["nsub"], gpu["h"], lwl=532e-9,
+                           inv_brems=True, Te=20.0, Z=1.0, coulomb_log=cl)
+        print("turbulent parity", cl, assert_parity(gpu, ref, coords))
+        assert np.all(np.diff(gpu["tau"], axis=0) >= 0)
+        assert 0 < np.mean(gpu["tau"][-1]) < 1
+
+
+@pytest.mark.parametrize("pd", ["x", "y"])
+def test_trace_other_probing_axes(orc, pd):
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import kolmogorov_field
+
+    L = 6e-3
+    ne, coords = kolmogorov_field(48, L, seed=5, n0=5e24, frac=0.5, l_min=0.5e-3)
+    s0 = Beam(2000, 2e-3, 2e-3, L / 2, probing_direction=pd, beam_type="circular",
+              seeded=True).s0
+    t = np.linspace(0, 1.5 * L / C, 4)
+    gpu = gpu_trace(ne, coords, s0, t, lwl=532e-9, inv_brems=True, Te=50.0, Z=2.0,
+                    probing_direction=pd)
+    ref = oracle_trace(orc, ne, coords, s0, gpu["nsub"], gpu["h"], lwl=532e-9, inv_brems=True,
+                       Te=50.0, Z=2.0)
+    print(pd, assert_parity(gpu, ref, coords))
+
+
+def test_trace_oblique_rays_outside_starts_and_side_exits(orc):
+    """Rays starting outside the box, entering through any face, missing it, leaving sideways."""
+    rng = np.random.default_rng(11)
+    L = 4e-3
+    from sloth_amd.fields import kolmogorov_field
+
+    ne, coords = kolmogorov_field(40, L, seed=7, n0=8e24, frac=0.4, l_min=0.4e-3)
+    n = 3000
+    # start on a sphere of radius 4 mm aimed at random points near the box
+    d = rng.normal(size=(3, n))
+    d /= np.linalg.norm(d, axis=0)
+    start = -4e-3 * d
+    target = rng.uniform(-3e-3, 3e-3, (3, n))
+    v = target - start
+    v = C * v / np.linalg.norm(v, axis=0)
+    s0 = np.vstack([start, v]).astype(np.float32)
+    t = np.linspace(0, 12e-3 / C, 3)
+    gpu = gpu_trace(ne, coords, s0, t, lwl=532e-9, inv_brems=True, Te=30.0, Z=1.0)
+    ref = oracle_trace(orc, ne, coords, s0, gpu["nsub"], gpu["h"], lwl=532e-9, inv_brems=True,
+                       Te=30.0, Z=1.0)
+    print("oblique", assert_parity(gpu, ref, coords, step_mismatch_frac=5e-3))
+    assert (gpu["steps"] == 0).any() and (gpu["steps"] > 0).any()
+
+
+def test_trace_empty_and_single_ray(orc):
+    import torch
+
+    from sloth_amd.core import engine
+
+    ne, coords = _sine(16, 2e-3)
+    g = engine.PackedGrid(ne, *coords, 2 * np.pi * C / 1064e-9)
+    t = np.array([0.0, 3e-3 / C])
+    r = engine.trace(g, np.zeros((6, 0), np.float32), t)
+    assert r.states.shape == (2, 6, 0) and int(r.total_steps.item()) == 0
+    one = rays(0.3e-3, -0.2e-3, coords[2][0])
+    gpu = gpu_trace(ne, coords, one, t, lwl=1064e-9)
+    ref = oracle_trace(orc, ne, coords, one, gpu["nsub"], gpu["h"], lwl=1064e-9)
+    assert_parity(gpu, ref, coords)
+    torch.cuda.synchronize()
+
+
+def test_trace_order_invariance_and_determinism():
+    from sloth_amd.core.beam import Beam
+
+    ne, coords = _sine(48, 10e-3)
+    s0 = Beam(5000, 3e-3, 1e-3, 5e-3, beam_type="circular", seeded=True).s0
+    t = np.linspace(0, 2.2 * 10e-3 / C, 3)
+    a = gpu_trace(ne, coords, s0, t, lwl=1064e-9, sort=True)
+    b = gpu_trace(ne, coords, s0, t, lwl=1064e-9, sort=False)
+    c = gpu_trace(ne, coords, s0, t, lwl=1064e-9, sort=True)
+    for k in ("states", "path", "steps"):
+        np.testing.assert_array_equal(a[k], b[k])
+        np.testing.assert_array_equal(a[k], c[k])
+
+
+def test_vacuum_is_exact():
+    ne = np.zeros((12, 12, 12), np.float32)
+    c = np.float32(np.linspace(-2e-3, 2e-3, 12))
+    rng = np.random.default_rng(12)
+    s0 = rays(rng.uniform(-3e-3, 3e-3, 500), rng.uniform(-3e-3, 3e-3, 500), -2e-3,
+              vx=rng.uniform(-1e6, 1e6, 500), vy=rng.uniform(-1e6, 1e6, 500))
+    T = 5e-3 / C
+    gpu = gpu_trace(ne, (c, c, c), s0, np.array([0.0, T]), lwl=1064e-9)
+    fin = gpu["states"][-1]
+    s = s0.astype(np.float64)
+    np.testing.assert_allclose(fin[:3], s[:3] + s[3:6] * T, rtol=0, atol=2e-9)
+    np.testing.assert_array_equal(fin[3:6], s[3:6])
