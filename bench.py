@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--rays", type=int, default=10_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the traced path runs on a single MI355X "
@@ -106,6 +108,14 @@ def main():
     from sloth_amd.shared.propagation import jones_device
 
     dev = torch.device("cuda", 0)
+    if args.copy_calibration_mib > 0:
+        # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # scale the trace kernel's counters to bytes
+        src = torch.ones(args.copy_calibration_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     L = 20e-3
     n = args.grid
     lwl = 532e-9
@@ -136,19 +146,23 @@ def main():
 
     log("[bench] warmup ...")
     for _ in range(args.warmup):
-        res, counts, _ = step()
+        step()
     torch.cuda.synchronize()
-    total_steps = int(res.total_steps.item())
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(args.steps)]
-    log(f"[bench] timing {args.steps} steps ({total_steps} ray-steps each) ...")
+    log(f"[bench] timing {args.steps} steps ...")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    results = []
     for k in range(args.steps):
-        res, counts, _ = step(evs[k])
+        results.append(step(evs[k]))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    assert int(res.total_steps.item()) == total_steps
+    res, counts, _ = results[-1]
+    per_pass = [int(r[0].total_steps.item()) for r in results]
+    total_steps = per_pass[0]
+    assert all(p == total_steps for p in per_pass), per_pass
+    del results
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
     ms_per_step = wall * 1e3 / args.steps
     value = total_steps / (ms_per_step * 1e-3)

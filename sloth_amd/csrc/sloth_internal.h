@@ -15,8 +15,9 @@ void clear_error();
 // Returns SLT_OK after a successful launch, else records the HIP error.
 int check_launch(const char* what);
 
-// Device view of a packed grid.  Indices fit in 32 bits: the host rejects
-// grids of 2^31 voxels or more.
+// Device view of a packed grid.  Indices fit in 32 bits and every stride and
+// extent in 24 (so voxel offsets use 24-bit multiplies): the host rejects
+// larger grids.
 struct DevGrid {
     const float4* __restrict__ vox;
     int n0, n1, n2;
@@ -36,6 +37,13 @@ inline int make_dev_grid(const void* packed, const slt_grid_desc* g, DevGrid* d)
     if (total >= (int64_t(1) << 31)) {
         set_error("grid of %lld voxels exceeds the 2^31 limit", (long long)total);
         return SLT_ERR_UNSUPPORTED;
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (g->stride[a] >= (int64_t(1) << 24) || g->n[a] >= (1 << 24)) {
+            set_error("grid axis %d: stride %lld / extent %d must stay below 2^24", a,
+                      (long long)g->stride[a], g->n[a]);
+            return SLT_ERR_UNSUPPORTED;
+        }
     }
     d->vox = static_cast<const float4*>(packed);
     d->n0 = g->n[0]; d->n1 = g->n[1]; d->n2 = g->n[2];

@@ -8,15 +8,19 @@
 //  * Rays arrive sorted along a Z-order curve of their transverse start
 //    position, so the 64 lanes of a wavefront sit in a few neighbouring grid
 //    columns and step through the same cells at the same time.
-//  * Each lane keeps the 8 corner voxels (8 x 16 B) of the cell it last
-//    touched in VGPRs and re-gathers only when a stage point moves to another
-//    cell.  A ray crosses a cell every ~2 steps along the probing axis (the
-//    fastest-varying axis of the packed layout), so the gather traffic per
-//    RK4 step is ~1/8 of the 4 x 8 corner loads the algorithm names.
-//  * Position and velocity are compensated fp32 pairs (hi + lo): the step
-//    increment is ~1e-3 of the position, so plain fp32 accumulation would
-//    lose ~1e-5 relative over 1000 steps.  tau and path length accumulate in
-//    fp64.  Everything else is fp32.
+//  * Each lane keeps the cell it last touched in VGPRs as the 8 coefficients
+//    of its trilinear polynomial (per channel: f = k0 + kx tx + ky ty + kz tz
+//    + kxy tx ty + kxz tx tz + kyz ty tz + kxyz tx ty tz), built from the 8
+//    corner voxels (8 x 16 B) when a stage point enters a new cell.  A ray
+//    crosses a cell every ~2 steps along the probing axis (the fastest-varying
+//    axis of the packed layout), so the gathers per RK4 step are ~1/8 of the
+//    4 x 8 corner loads the algorithm names, and an evaluation is 7 FMAs per
+//    channel.
+//  * Position = fp64 base + fp32 offset: the offset takes each step's
+//    increment and is folded into the base every 8 steps, so rounding stays at
+//    the offset's (small) magnitude.  Velocity = fixed start velocity + fp32
+//    change.  tau and path length accumulate in fp64.  All RHS arithmetic is
+//    fp32.
 //  * Steps whose straight segment cannot reach the box are exact linear moves
 //    (the RHS is identically 0 there); a ray outside the box skips all of them
 //    at once.  Only steps with a stage point inside the box are counted.
@@ -45,22 +49,46 @@ struct TraceK {
     int nblocks;
 };
 
-struct Corners {
-    int cell;
-    float4 c[8];
+// Trilinear polynomial of the cached cell, one float4 (gx, gy, gz, kappa) per
+// coefficient.
+struct Cell {
+    int id;  // voxel index of corner (i, j, k); -1 = none cached
+    float4 k0, kx, ky, kz, kxy, kxz, kyz, kxyz;
 };
 
-__device__ __forceinline__ void load_corners(const DevGrid& G, int cell, Corners& cc) {
-    const float4* __restrict__ p = G.vox + cell;
-    cc.c[0] = p[0];
-    cc.c[1] = p[G.s2];
-    cc.c[2] = p[G.s1];
-    cc.c[3] = p[G.s1 + G.s2];
-    cc.c[4] = p[G.s0];
-    cc.c[5] = p[G.s0 + G.s2];
-    cc.c[6] = p[G.s0 + G.s1];
-    cc.c[7] = p[G.s0 + G.s1 + G.s2];
-    cc.cell = cell;
+__device__ __forceinline__ float4 f4sub(float4 a, float4 b) {
+    return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+}
+
+__device__ __forceinline__ void load_cell(const DevGrid& G, int id, Cell& c) {
+    const float4* __restrict__ p = G.vox + id;
+    const float4 c000 = p[0];
+    const float4 c001 = p[G.s2];
+    const float4 c010 = p[G.s1];
+    const float4 c011 = p[G.s1 + G.s2];
+    const float4 c100 = p[G.s0];
+    const float4 c101 = p[G.s0 + G.s2];
+    const float4 c110 = p[G.s0 + G.s1];
+    const float4 c111 = p[G.s0 + G.s1 + G.s2];
+    const float4 dz00 = f4sub(c001, c000), dz01 = f4sub(c011, c010);
+    const float4 dz10 = f4sub(c101, c100), dz11 = f4sub(c111, c110);
+    c.k0 = c000;
+    c.kz = dz00;
+    c.kx = f4sub(c100, c000);
+    c.ky = f4sub(c010, c000);
+    c.kxy = f4sub(f4sub(c110, c010), c.kx);
+    c.kyz = f4sub(dz01, dz00);
+    c.kxz = f4sub(dz10, dz00);
+    c.kxyz = f4sub(f4sub(dz11, dz10), c.kyz);
+    c.id = id;
+}
+
+__device__ __forceinline__ float tri(float tx, float ty, float tz, float k0, float kx, float ky,
+                                     float kz, float kxy, float kxz, float kyz, float kxyz) {
+    const float a = fmaf(tx, kxyz, kyz);
+    const float b = fmaf(ty, a, fmaf(tx, kxz, kz));
+    const float c = fmaf(tx, kxy, ky);
+    return fmaf(tz, b, fmaf(ty, c, fmaf(tx, kx, k0)));
 }
 
 __device__ __forceinline__ bool in_box(const DevGrid& G, float x, float y, float z) {
@@ -68,11 +96,12 @@ __device__ __forceinline__ bool in_box(const DevGrid& G, float x, float y, float
            (z <= G.hi2);
 }
 
-// Interpolated (gx, gy, gz, kappa) at (x, y, z); zero and false outside the box.
-template <bool KAPPA>
-__device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, float z,
-                                         Corners& cc, float4& out) {
-    if (!in_box(G, x, y, z)) {
+// Interpolated (gx, gy, gz, kappa) at (x, y, z); zero and false outside the
+// closed box.  CHECK = false when the caller already knows the point is inside.
+template <bool KAPPA, bool CHECK>
+__device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, float z, Cell& cc,
+                                         float4& out) {
+    if (CHECK && !in_box(G, x, y, z)) {
         out = make_float4(0.f, 0.f, 0.f, 0.f);
         return false;
     }
@@ -82,36 +111,21 @@ __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, flo
     const int i0 = min((int)u0, G.n0 - 2);  // u >= 0 inside the box: truncation == floor
     const int i1 = min((int)u1, G.n1 - 2);
     const int i2 = min((int)u2, G.n2 - 2);
-    const float t0 = u0 - (float)i0;
-    const float t1 = u1 - (float)i1;
-    const float t2 = u2 - (float)i2;
-    const int cell = i0 * G.s0 + i1 * G.s1 + i2 * G.s2;
-    if (cell != cc.cell) load_corners(G, cell, cc);
-    const float a0 = 1.f - t0, a1 = 1.f - t1, a2 = 1.f - t2;
-    const float w00 = a0 * a1, w01 = a0 * t1, w10 = t0 * a1, w11 = t0 * t1;
-    const float w[8] = {w00 * a2, w00 * t2, w01 * a2, w01 * t2,
-                        w10 * a2, w10 * t2, w11 * a2, w11 * t2};
-    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        r.x = fmaf(cc.c[e].x, w[e], r.x);
-        r.y = fmaf(cc.c[e].y, w[e], r.y);
-        r.z = fmaf(cc.c[e].z, w[e], r.z);
-        if (KAPPA) r.w = fmaf(cc.c[e].w, w[e], r.w);
-    }
-    out = r;
+    const float tx = u0 - (float)i0;
+    const float ty = u1 - (float)i1;
+    const float tz = u2 - (float)i2;
+    const int id = (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2));
+    if (id != cc.id) load_cell(G, id, cc);
+    out.x = tri(tx, ty, tz, cc.k0.x, cc.kx.x, cc.ky.x, cc.kz.x, cc.kxy.x, cc.kxz.x, cc.kyz.x,
+                cc.kxyz.x);
+    out.y = tri(tx, ty, tz, cc.k0.y, cc.kx.y, cc.ky.y, cc.kz.y, cc.kxy.y, cc.kxz.y, cc.kyz.y,
+                cc.kxyz.y);
+    out.z = tri(tx, ty, tz, cc.k0.z, cc.kx.z, cc.ky.z, cc.kz.z, cc.kxy.z, cc.kxz.z, cc.kyz.z,
+                cc.kxyz.z);
+    out.w = KAPPA ? tri(tx, ty, tz, cc.k0.w, cc.kx.w, cc.ky.w, cc.kz.w, cc.kxy.w, cc.kxz.w,
+                        cc.kyz.w, cc.kxyz.w)
+                  : 0.f;
     return true;
-}
-
-// (hi, lo) += a + b, where a is the main increment and b a small correction.
-__device__ __forceinline__ void comp_add(float& hi, float& lo, float a, float b) {
-    const float s = hi + a;
-    const float bp = s - hi;
-    const float e = (hi - (s - bp)) + (a - bp);  // exact error of hi + a (TwoSum)
-    float l = lo + (e + b);
-    const float s2 = s + l;
-    lo = l - (s2 - s);
-    hi = s2;
 }
 
 // Number of whole steps of length h (at most kmax) that a ray outside the box
@@ -152,20 +166,26 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceK K) {
     if (valid) {
         const int64_t n = K.n;
         const int64_t ray = K.order ? (int64_t)K.order[t] : t;
-        float rh[3], rl[3] = {0.f, 0.f, 0.f}, vh[3], vl[3] = {0.f, 0.f, 0.f};
+        double rb[3];           // position base [m]
+        float rbf[3];           // rb rounded to fp32
+        float off[3] = {0.f, 0.f, 0.f};  // position offset since the last fold
+        float v0[3];            // start velocity [m/s]
+        float dv[3] = {0.f, 0.f, 0.f};   // velocity change since the start
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            rh[c] = K.s0[c * n + ray];
-            vh[c] = K.s0[(3 + c) * n + ray];
+            rbf[c] = K.s0[c * n + ray];
+            rb[c] = (double)rbf[c];
+            v0[c] = K.s0[(3 + c) * n + ray];
         }
         double tau = 0.0, path = 0.0;
-        Corners cc;
-        cc.cell = -1;
+        Cell cc;
+        cc.id = -1;
+        uint32_t since_fold = 0;
         if (K.save_every) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                K.save_state[c * n + ray] = rh[c];
-                K.save_state[(3 + c) * n + ray] = vh[c];
+                K.save_state[c * n + ray] = rbf[c];
+                K.save_state[(3 + c) * n + ray] = v0[c];
             }
             if (K.save_tau) K.save_tau[ray] = 0.0;
         }
@@ -177,17 +197,23 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceK K) {
             const float hh6 = h * h6;
             int k = 0;
             while (k < nsub) {
-                if (!in_box(G, rh[0], rh[1], rh[2])) {
+                float rh[3], vh[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    rh[c] = rbf[c] + off[c];
+                    vh[c] = v0[c] + dv[c];
+                }
+                const bool in1 = in_box(G, rh[0], rh[1], rh[2]);
+                if (!in1) {
                     const int m = ballistic_steps(G, rh, vh, h, nsub - k);
                     if (m > 0) {
                         const double T = (double)m * (double)h;
                         double sp2 = 0.0;
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
-                            const double vc = (double)vh[c] + (double)vl[c];
-                            const double rc = (double)rh[c] + (double)rl[c] + T * vc;
-                            rh[c] = (float)rc;
-                            rl[c] = (float)(rc - (double)rh[c]);
+                            const double vc = (double)v0[c] + (double)dv[c];
+                            rb[c] += T * vc;
+                            rbf[c] = (float)rb[c];
                             sp2 += vc * vc;
                         }
                         if (PATH) path += T * sqrt(sp2);
@@ -196,52 +222,56 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceK K) {
                     }
                 }
                 // One RK4 step.
-                float4 A1, A2, A3, A4;
-                bool in = eval_rhs<KAPPA>(G, rh[0], rh[1], rh[2], cc, A1);
+                float4 A1 = make_float4(0.f, 0.f, 0.f, 0.f), A2, A3, A4;
+                bool in = in1;
+                if (in1) eval_rhs<KAPPA, false>(G, rh[0], rh[1], rh[2], cc, A1);
                 const float v2x = fmaf(hh, A1.x, vh[0]), v2y = fmaf(hh, A1.y, vh[1]),
                             v2z = fmaf(hh, A1.z, vh[2]);
-                in |= eval_rhs<KAPPA>(G, fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]),
-                                      fmaf(hh, vh[2], rh[2]), cc, A2);
+                in |= eval_rhs<KAPPA, true>(G, fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]),
+                                            fmaf(hh, vh[2], rh[2]), cc, A2);
                 const float v3x = fmaf(hh, A2.x, vh[0]), v3y = fmaf(hh, A2.y, vh[1]),
                             v3z = fmaf(hh, A2.z, vh[2]);
-                in |= eval_rhs<KAPPA>(G, fmaf(hh, v2x, rh[0]), fmaf(hh, v2y, rh[1]),
-                                      fmaf(hh, v2z, rh[2]), cc, A3);
+                in |= eval_rhs<KAPPA, true>(G, fmaf(hh, v2x, rh[0]), fmaf(hh, v2y, rh[1]),
+                                            fmaf(hh, v2z, rh[2]), cc, A3);
                 const float v4x = fmaf(h, A3.x, vh[0]), v4y = fmaf(h, A3.y, vh[1]),
                             v4z = fmaf(h, A3.z, vh[2]);
-                in |= eval_rhs<KAPPA>(G, fmaf(h, v3x, rh[0]), fmaf(h, v3y, rh[1]),
-                                      fmaf(h, v3z, rh[2]), cc, A4);
+                in |= eval_rhs<KAPPA, true>(G, fmaf(h, v3x, rh[0]), fmaf(h, v3y, rh[1]),
+                                            fmaf(h, v3z, rh[2]), cc, A4);
                 if (PATH) {
-                    const float s1 = sqrtf(vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2]);
-                    const float s2 = sqrtf(v2x * v2x + v2y * v2y + v2z * v2z);
-                    const float s3 = sqrtf(v3x * v3x + v3y * v3y + v3z * v3z);
-                    const float s4 = sqrtf(v4x * v4x + v4y * v4y + v4z * v4z);
+                    const float s1 = __builtin_amdgcn_sqrtf(
+                        fmaf(vh[0], vh[0], fmaf(vh[1], vh[1], vh[2] * vh[2])));
+                    const float s2 = __builtin_amdgcn_sqrtf(fmaf(v2x, v2x, fmaf(v2y, v2y, v2z * v2z)));
+                    const float s3 = __builtin_amdgcn_sqrtf(fmaf(v3x, v3x, fmaf(v3y, v3y, v3z * v3z)));
+                    const float s4 = __builtin_amdgcn_sqrtf(fmaf(v4x, v4x, fmaf(v4y, v4y, v4z * v4z)));
                     path += (double)(h6 * (s1 + 2.f * (s2 + s3) + s4));
                 }
                 // r += h v + h^2/6 (A1 + A2 + A3)   (= h/6 (v1 + 2 v2 + 2 v3 + v4))
                 // v += h/6 (A1 + 2 A2 + 2 A3 + A4)
-                const float S[3] = {A1.x + A2.x + A3.x, A1.y + A2.y + A3.y, A1.z + A2.z + A3.z};
-                const float D[3] = {A1.x + 2.f * (A2.x + A3.x) + A4.x,
-                                    A1.y + 2.f * (A2.y + A3.y) + A4.y,
-                                    A1.z + 2.f * (A2.z + A3.z) + A4.z};
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const float p = h * vh[c];
-                    const float pe = fmaf(h, vh[c], -p);  // exact rounding error of h*vh
-                    const float corr = fmaf(hh6, S[c], fmaf(h, vl[c], pe));
-                    comp_add(rh[c], rl[c], p, corr);
-                    comp_add(vh[c], vl[c], h6 * D[c], 0.f);
-                }
+                off[0] += fmaf(h, vh[0], hh6 * (A1.x + A2.x + A3.x));
+                off[1] += fmaf(h, vh[1], hh6 * (A1.y + A2.y + A3.y));
+                off[2] += fmaf(h, vh[2], hh6 * (A1.z + A2.z + A3.z));
+                dv[0] += h6 * (A1.x + 2.f * (A2.x + A3.x) + A4.x);
+                dv[1] += h6 * (A1.y + 2.f * (A2.y + A3.y) + A4.y);
+                dv[2] += h6 * (A1.z + 2.f * (A2.z + A3.z) + A4.z);
                 if (KAPPA) tau += (double)(h6 * (A1.w + 2.f * (A2.w + A3.w) + A4.w));
                 nsteps += in ? 1u : 0u;
                 ++k;
+                if ((++since_fold & 7u) == 0) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        rb[c] += (double)off[c];
+                        off[c] = 0.f;
+                        rbf[c] = (float)rb[c];
+                    }
+                }
             }
             if (K.save_every || seg == K.nseg - 1) {
                 const int64_t slot = K.save_every ? (int64_t)(seg + 1) : 0;
                 float* base = K.save_state + slot * 6 * n;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    base[c * n + ray] = rh[c];
-                    base[(3 + c) * n + ray] = vh[c];
+                    base[c * n + ray] = (float)(rb[c] + (double)off[c]);
+                    base[(3 + c) * n + ray] = v0[c] + dv[c];
                 }
                 if (K.save_tau) K.save_tau[slot * n + ray] = tau;
             }
