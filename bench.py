@@ -82,6 +82,79 @@ def cpu_baseline(ne, coords, s0, nsub, h, omega, target_s):
                       f"schedule, FP64 C oracle with OpenMP, {dt:.1f} s"}
 
 
+def serving_mix(n_requests, seed=12345):
+    """The serving mix: grid side from {64,128,192,256,384}, rays log-uniform in [1e4, 4e6],
+    532 or 1064 nm, sine field (no absorption) or Kolmogorov field (absorption)."""
+    from sloth_amd.serving import BeamSpec, FieldSpec, RequestSpec
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n_requests):
+        side = int(rng.choice([64, 128, 192, 256, 384]))
+        rays = int(np.exp(rng.uniform(np.log(1e4), np.log(4e6))))
+        lwl = float(rng.choice([532e-9, 1064e-9]))
+        if rng.random() < 0.5:
+            f = FieldSpec(kind="sine", side=side, length=10e-3, seed=int(rng.integers(1 << 30)),
+                          n0=1e24, frac=0.1, wavelength=2e-3)
+            inv = False
+        else:
+            f = FieldSpec(kind="kolmogorov", side=side, length=10e-3,
+                          seed=int(rng.integers(1 << 30)), n0=2e24, frac=0.3, l_min=0.2e-3)
+            inv = True
+        reqs.append(RequestSpec(rid=i, field=f, beam=BeamSpec(n_rays=rays, half_width=4e-3,
+                                                               seed=int(rng.integers(1 << 30))),
+                                lwl=lwl, inv_brems=inv, Te=20.0, Z=1.0, coulomb_log=10.0,
+                                bins=256))
+    return reqs
+
+
+def serving_bench(n_requests):
+    from sloth_amd.serving import ServingEngine, run_offline
+
+    from sloth_amd.serving import BeamSpec, FieldSpec, RequestSpec
+
+    reqs = serving_mix(n_requests)
+    # rocFFT builds its kernels at run time per transform size: warm every grid side
+    # and field kind once so neither leg below pays for compilation
+    warm = [RequestSpec(rid=i, field=FieldSpec(kind=k, side=s, length=10e-3),
+                        beam=BeamSpec(n_rays=1000), inv_brems=(k == "kolmogorov"))
+            for i, (k, s) in enumerate((k, s) for k in ("sine", "kolmogorov")
+                                       for s in (64, 128, 192, 256, 384))]
+    run_offline(warm)
+    ServingEngine(max_slots=len(warm)).run(warm, realtime=False)
+    _, off = run_offline(reqs)
+    log(f"[serving] offline batch: {off['ray_steps_per_s']:.3e} ray-steps/s over "
+        f"{off['wall_s']:.2f} s")
+    eng = ServingEngine(max_slots=32)
+    _, sat = eng.run(serving_mix(n_requests), realtime=False)
+    log(f"[serving] saturated: {sat['ray_steps_per_s']:.3e} ray-steps/s, {sat['batches']} batches")
+    capacity = n_requests / off["wall_s"]
+    rate = 0.7 * capacity
+    mix = serving_mix(n_requests)
+    rng = np.random.default_rng(777)
+    t = 0.0
+    for r in mix:
+        t += rng.exponential(1.0 / rate)
+        r.arrival = t
+    _, poi = ServingEngine(max_slots=32).run(mix, realtime=True)
+    log(f"[serving] poisson {rate:.1f} req/s: p50 {poi['p50_latency_s'] * 1e3:.1f} ms, "
+        f"p99 {poi['p99_latency_s'] * 1e3:.1f} ms")
+    return {
+        "requests": n_requests,
+        "mix": "side {64,128,192,256,384}, rays log-uniform [1e4,4e6], 532/1064 nm, "
+               "sine (no absorption) or Kolmogorov (absorption, CL 10), 10 mm box, 256^2 detector",
+        "offline_ray_steps_per_s": off["ray_steps_per_s"],
+        "saturated_ray_steps_per_s": sat["ray_steps_per_s"],
+        "saturated_over_offline": sat["ray_steps_per_s"] / off["ray_steps_per_s"],
+        "poisson_rate_hz": rate,
+        "poisson_load": 0.7,
+        "p50_latency_s": poi["p50_latency_s"],
+        "p99_latency_s": poi["p99_latency_s"],
+        "poisson_ray_steps_per_s": poi["ray_steps_per_s"],
+        "total_ray_steps": off["ray_steps"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +164,8 @@ def main():
     ap.add_argument("--rays", type=int, default=10_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=48,
+                    help="requests in the serving mix (0 skips the serving leg)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (PMC calibration)")
     args = ap.parse_args()
@@ -182,6 +257,11 @@ def main():
         nsub, h = engine.schedule(save_times, engine.nominal_step(grid, 0.5))
         cpu = cpu_baseline(ne, coords, s0_host, nsub, h, omega, args.cpu_seconds)
         log(f"[bench] cpu {cpu['value']:.3e} ray-steps/s on {cpu['cores']} cores")
+    serving = None
+    if args.serving_requests > 0:
+        log("[bench] serving mix ...")
+        torch.cuda.empty_cache()
+        serving = serving_bench(args.serving_requests)
     out = {
         "metric": "ray-steps/s (512^3 grid, 1e7 rays)",
         "value": value,
@@ -202,6 +282,7 @@ def main():
                    "accumulators": "fp64 tau and path"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]

@@ -174,6 +174,69 @@ int slt_histogram2d(const void* x, int64_t x_stride, const void* y, int64_t y_st
 int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int64_t* keys,
                     slt_stream_t stream);
 
+/* ------------------------------------------------------------------------- */
+/* Serving: resumable ray state and multi-request step batches.              */
+/* No reference counterpart (the reference traces one request per call).     */
+/* ------------------------------------------------------------------------- */
+
+#define SLT_MAX_JOBS 64
+
+/* Bytes of resumable state for n rays (84 B per ray). */
+int64_t slt_ray_state_bytes(int64_t n_rays);
+
+/*
+ * Initialise resumable state: row t takes ray order[t] of s0 (6 x n, row stride
+ * n), or ray t if order is NULL.
+ */
+int slt_ray_state_init(void* state, int64_t n_rays, const float* s0, const int32_t* order,
+                       slt_stream_t stream);
+
+/* One request's rays and the step range a step batch advances them through. */
+typedef struct slt_job {
+    const void* packed;        /* packed grid (slt_grid_pack)                        */
+    slt_grid_desc grid;
+    void* state;               /* slt_ray_state_bytes(n_rays) bytes (slt_ray_state_init) */
+    const int32_t* order;      /* ray index of state row t (the init order), or NULL */
+    int64_t n_rays;
+    const int32_t* seg_nsub;   /* save schedule, as in slt_trace_args                */
+    const float* seg_h;
+    int32_t n_seg;
+    int32_t reserved;
+    int64_t sched_steps;       /* sum of seg_nsub                                    */
+    int64_t step_begin;        /* advance every ray from this global step ...        */
+    int64_t step_end;          /* ... to this one; == sched_steps finishes the job   */
+    float* out_state;          /* 6 x n_rays final state in caller order (finishing) */
+    double* out_tau;           /* n_rays final tau, or NULL                          */
+    double* out_path;          /* n_rays final path length, or NULL                  */
+    uint32_t* out_steps;       /* n_rays in-box steps, or NULL                       */
+    unsigned long long* step_counter; /* += in-box steps taken by this batch, or NULL */
+} slt_job;
+
+typedef struct slt_job_table {
+    int32_t n_jobs;
+    int32_t n_units;                      /* filled by slt_job_table_prepare */
+    int32_t unit_begin[SLT_MAX_JOBS + 1]; /* filled by slt_job_table_prepare */
+    slt_job jobs[SLT_MAX_JOBS];
+} slt_job_table;
+
+/*
+ * Host-only: validate every job (grid, pointers, step ranges) and fill
+ * n_units / unit_begin (one unit = 256 rays).  Call before copying the table
+ * to the device.
+ */
+int slt_job_table_prepare(slt_job_table* host_table);
+
+/*
+ * One step batch: advance every job of a device-resident, prepared table by
+ * its [step_begin, step_end) range, in one persistent launch of `blocks`
+ * workgroups (a positive multiple of 8).  The kernel arguments do not depend
+ * on the table's contents, so a launch can be captured in a HIP graph once and
+ * replayed after each table update.  A request advanced in several batches
+ * gives the same result as one slt_trace call (up to fp64 rounding of the
+ * straight-line moves outside the grid, which are split at batch ends).
+ */
+int slt_advance(const slt_job_table* device_table, int32_t blocks, slt_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,7 +20,10 @@ SLT_ERR_UNSUPPORTED = -3
 EXPORTED = (
     "slt_version", "slt_last_error", "slt_grid_desc_init", "slt_grid_bytes", "slt_grid_pack",
     "slt_kappa", "slt_trace", "slt_jones", "slt_histogram2d", "slt_morton_keys",
+    "slt_ray_state_bytes", "slt_ray_state_init", "slt_job_table_prepare", "slt_advance",
 )
+
+SLT_MAX_JOBS = 64
 
 
 class GridDesc(ctypes.Structure):
@@ -49,6 +52,37 @@ class TraceArgs(ctypes.Structure):
         ("path", ctypes.c_void_p),
         ("steps", ctypes.c_void_p),
         ("total_steps", ctypes.c_void_p),
+    ]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [
+        ("packed", ctypes.c_void_p),
+        ("grid", GridDesc),
+        ("state", ctypes.c_void_p),
+        ("order", ctypes.c_void_p),
+        ("n_rays", ctypes.c_int64),
+        ("seg_nsub", ctypes.c_void_p),
+        ("seg_h", ctypes.c_void_p),
+        ("n_seg", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("sched_steps", ctypes.c_int64),
+        ("step_begin", ctypes.c_int64),
+        ("step_end", ctypes.c_int64),
+        ("out_state", ctypes.c_void_p),
+        ("out_tau", ctypes.c_void_p),
+        ("out_path", ctypes.c_void_p),
+        ("out_steps", ctypes.c_void_p),
+        ("step_counter", ctypes.c_void_p),
+    ]
+
+
+class JobTable(ctypes.Structure):
+    _fields_ = [
+        ("n_jobs", ctypes.c_int32),
+        ("n_units", ctypes.c_int32),
+        ("unit_begin", ctypes.c_int32 * (SLT_MAX_JOBS + 1)),
+        ("jobs", Job * SLT_MAX_JOBS),
     ]
 
 
@@ -82,6 +116,11 @@ def lib():
     L.slt_jones.argtypes = [vp, i64, i64, i32, vp, vp, vp, vp]
     L.slt_histogram2d.argtypes = [vp, i64, vp, i64, i32, i64, vp, vp, i32, vp, i32, vp, vp, vp, vp]
     L.slt_morton_keys.argtypes = [vp, i64, ctypes.POINTER(GridDesc), vp, vp]
+    L.slt_ray_state_bytes.argtypes = [i64]
+    L.slt_ray_state_bytes.restype = i64
+    L.slt_ray_state_init.argtypes = [vp, i64, vp, vp, vp]
+    L.slt_job_table_prepare.argtypes = [ctypes.POINTER(JobTable)]
+    L.slt_advance.argtypes = [vp, i32, vp]
     for name in EXPORTED:
         getattr(L, name).restype = getattr(L, name).restype or ctypes.c_int
     _lib = L

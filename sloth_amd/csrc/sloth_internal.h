@@ -55,6 +55,19 @@ inline int make_dev_grid(const void* packed, const slt_grid_desc* g, DevGrid* d)
     return SLT_OK;
 }
 
+// Device-side fill of a DevGrid from a descriptor the host already checked
+// (slt_job_table_prepare).
+__host__ __device__ inline void make_dev_grid_device(const void* packed, const slt_grid_desc* g,
+                                                     DevGrid* d) {
+    d->vox = static_cast<const float4*>(packed);
+    d->n0 = g->n[0]; d->n1 = g->n[1]; d->n2 = g->n[2];
+    d->s0 = (int)g->stride[0]; d->s1 = (int)g->stride[1]; d->s2 = (int)g->stride[2];
+    d->lo0 = g->lo[0]; d->lo1 = g->lo[1]; d->lo2 = g->lo[2];
+    d->hi0 = g->hi[0]; d->hi1 = g->hi[1]; d->hi2 = g->hi[2];
+    d->id0 = g->inv_d[0]; d->id1 = g->inv_d[1]; d->id2 = g->inv_d[2];
+    d->nchan = g->nchan;
+}
+
 // Number of XCDs on MI355X; workgroups are dispatched to XCDs round-robin by
 // workgroup id, so block b runs on XCD (b % kXcds).
 constexpr int kXcds = 8;

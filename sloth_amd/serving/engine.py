@@ -1,0 +1,365 @@
+"""Continuous-batching ray-trace serving on one GPU.
+
+Requests (``RequestSpec``) arrive over time.  The engine
+
+* admits them FIFO while a slot is free and their HBM footprint fits the
+  budget (a request larger than the whole budget is rejected at once),
+* sets each admitted request up on a side stream (field synthesis, k1 pack,
+  beam, Morton order, resumable ray state) while earlier requests keep
+  running,
+* advances all set-up requests together in *step batches*: one persistent
+  ``slt_advance`` launch per batch walks a device-resident job table in which
+  every request has its own step range; the launch is captured once in a HIP
+  graph and replayed after each table update,
+* finalises a request (Jones vector, exp(-tau), detector image) on the
+  compute stream as soon as its last batch is enqueued, and reports its
+  latency from GPU timestamps.
+
+A request's result does not depend on what it shares batches with: every ray
+runs the same per-ray integration as the offline ``slt_trace`` path, only
+split at batch boundaries.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .. import _native as N
+from ..core import engine
+from ..processing.diagnostics import histogram2d_device
+from ..shared.propagation import jones_device
+from .requests import RequestSpec, materialize
+
+
+# ---------------------------------------------------------------------------
+# Admission (host only)
+# ---------------------------------------------------------------------------
+
+class Admission:
+    """FIFO admission by free slots and HBM footprint."""
+
+    def __init__(self, budget_bytes: int, max_slots: int):
+        if max_slots < 1:
+            raise ValueError("max_slots must be >= 1")
+        self.budget = int(budget_bytes)
+        self.max_slots = int(max_slots)
+        self.used = 0
+        self.slots = 0
+
+    def admit(self, queue: deque, now: float):
+        """Pop the arrived head-of-queue requests that fit.  Returns (admitted, rejected)."""
+        admitted, rejected = [], []
+        while queue and queue[0].arrival <= now:
+            req = queue[0]
+            fp = req.footprint_bytes()
+            if fp > self.budget:
+                rejected.append(queue.popleft())
+                continue
+            if self.slots >= self.max_slots or self.used + fp > self.budget:
+                break
+            queue.popleft()
+            self.used += fp
+            self.slots += 1
+            admitted.append(req)
+        return admitted, rejected
+
+    def release(self, req: RequestSpec):
+        self.used -= req.footprint_bytes()
+        self.slots -= 1
+        assert self.used >= 0 and self.slots >= 0
+
+
+@dataclass
+class Response:
+    rid: int
+    status: str                     # "ok" | "rejected" | "error"
+    message: str = ""
+    rf: torch.Tensor | None = None  # (4, N) float32
+    amp: torch.Tensor | None = None  # (N,) float64 exp(-tau)
+    image: torch.Tensor | None = None  # (bins, bins) counts or exp(-tau)-weighted sums
+    path: torch.Tensor | None = None
+    ray_steps: int = 0
+    arrival: float = 0.0
+    completed: float = math.nan     # seconds since the run started (GPU clock)
+    n_rays: int = 0
+
+    @property
+    def latency(self) -> float:
+        return self.completed - self.arrival
+
+
+@dataclass
+class _Active:
+    req: RequestSpec
+    mat: object
+    order: torch.Tensor
+    state: torch.Tensor
+    seg_nsub: torch.Tensor
+    seg_h: torch.Tensor
+    sched: int
+    inbox_estimate: int
+    out_state: torch.Tensor
+    out_tau: torch.Tensor
+    out_path: torch.Tensor
+    counter: torch.Tensor
+    ready: torch.cuda.Event
+    cursor: int = 0
+    finished: bool = False
+    done: torch.cuda.Event | None = None
+    result: dict = field(default_factory=dict)
+
+
+def _f(t):
+    return t.data_ptr() if t is not None else None
+
+
+class ServingEngine:
+    def __init__(self, *, max_slots: int = 32, hbm_budget: int | None = None,
+                 blocks: int = 1024, batch_work: float = 2e9, use_graph: bool = True,
+                 step_fraction: float = 0.5, keep_outputs: bool = True, device=None):
+        self.dev = device if device is not None else N.require_device()
+        if hbm_budget is None:
+            free, _total = torch.cuda.mem_get_info(self.dev)
+            hbm_budget = int(0.8 * free)
+        self.max_slots = min(int(max_slots), N.SLT_MAX_JOBS)
+        self.budget = int(hbm_budget)
+        if blocks < 8 or blocks % 8:
+            raise ValueError("blocks must be a positive multiple of 8")
+        self.blocks = int(blocks)
+        self.batch_work = float(batch_work)
+        self.step_fraction = step_fraction
+        self.keep_outputs = keep_outputs
+        self.main = torch.cuda.Stream(self.dev)
+        self.setup_stream = torch.cuda.Stream(self.dev)
+        tb = ctypes.sizeof(N.JobTable)
+        self.table_dev = torch.zeros(tb, dtype=torch.uint8, device=self.dev)
+        self._ring = [torch.zeros(tb, dtype=torch.uint8).pin_memory() for _ in range(4)]
+        self._ring_ev = [None] * 4
+        self._ring_i = 0
+        self.graph = None
+        self.use_graph = use_graph
+        self.batches = 0
+        L = N.lib()
+        empty = N.JobTable()
+        N.check(L.slt_job_table_prepare(ctypes.byref(empty)))
+        with torch.cuda.stream(self.main):
+            self._upload(empty)
+            N.check(L.slt_advance(self.table_dev.data_ptr(), self.blocks, N.stream_handle()))
+        self.main.synchronize()
+        if use_graph:
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                N.check(L.slt_advance(self.table_dev.data_ptr(), self.blocks, N.stream_handle()))
+            torch.cuda.synchronize(self.dev)
+
+    # -- table upload through a ring of pinned buffers ------------------------
+    def _upload(self, table: N.JobTable):
+        i = self._ring_i
+        self._ring_i = (i + 1) % len(self._ring)
+        ev = self._ring_ev[i]
+        if ev is not None:
+            ev.synchronize()
+        buf = self._ring[i]
+        ctypes.memmove(buf.data_ptr(), ctypes.addressof(table), ctypes.sizeof(table))
+        self.table_dev.copy_(buf, non_blocking=True)
+        e = torch.cuda.Event()
+        e.record(torch.cuda.current_stream())
+        self._ring_ev[i] = e
+
+    # -- request setup on the side stream -------------------------------------
+    def _setup(self, req: RequestSpec) -> _Active:
+        with torch.cuda.stream(self.setup_stream):
+            mat = materialize(req, self.dev)
+            n = mat.rays.shape[1]
+            nsub, h = engine.schedule(mat.save_times, engine.nominal_step(mat.grid, self.step_fraction))
+            order = engine.ray_order(mat.grid, mat.rays)
+            state = torch.empty(int(N.lib().slt_ray_state_bytes(n)), dtype=torch.uint8,
+                                device=self.dev)
+            N.check(N.lib().slt_ray_state_init(state.data_ptr(), n, mat.rays.data_ptr(),
+                                               order.data_ptr(), N.stream_handle()))
+            seg_nsub = torch.from_numpy(nsub).to(self.dev)
+            seg_h = torch.from_numpy(h.astype(np.float32)).to(self.dev)
+            out_state = torch.empty((6, n), dtype=torch.float32, device=self.dev)
+            out_tau = torch.zeros(n, dtype=torch.float64, device=self.dev)
+            out_path = torch.empty(n, dtype=torch.float64, device=self.dev)
+            counter = torch.zeros(1, dtype=torch.int64, device=self.dev)
+            ready = torch.cuda.Event()
+            ready.record(self.setup_stream)
+        for t in (mat.grid.voxels, state, order, seg_nsub, seg_h, out_state, out_tau, out_path,
+                  counter):
+            t.record_stream(self.main)
+        sched = int(nsub.astype(np.int64).sum())
+        # steps a ray needs to cross the box along the probing axis; after that the
+        # remaining steps are straight-line moves and go in one batch
+        ax = mat.grid.axis
+        span = float(mat.grid.coords[ax][-1]) - float(mat.grid.coords[ax][0])
+        inbox = int(math.ceil(span / (299792458.0 * float(h[0])))) + 4 if h.size and h[0] > 0 else sched
+        return _Active(req, mat, order, state, seg_nsub, seg_h, sched, min(inbox, sched),
+                       out_state, out_tau, out_path, counter, ready)
+
+    def _job(self, a: _Active, begin: int, end: int) -> N.Job:
+        j = N.Job()
+        j.packed = a.mat.grid.voxels.data_ptr()
+        j.grid = a.mat.grid.desc
+        j.state = a.state.data_ptr()
+        j.order = a.order.data_ptr()
+        j.n_rays = a.mat.rays.shape[1]
+        j.seg_nsub = a.seg_nsub.data_ptr()
+        j.seg_h = a.seg_h.data_ptr()
+        j.n_seg = a.seg_nsub.numel()
+        j.sched_steps = a.sched
+        j.step_begin = begin
+        j.step_end = end
+        j.out_state = a.out_state.data_ptr()
+        j.out_tau = _f(a.out_tau) if a.mat.grid.inv_brems else None
+        j.out_path = a.out_path.data_ptr()
+        j.out_steps = None
+        j.step_counter = a.counter.data_ptr()
+        return j
+
+    def _finalize(self, a: _Active):
+        inv = a.mat.grid.inv_brems
+        rf, amp = jones_device(a.out_state, a.mat.grid.probing_direction,
+                               a.out_tau if inv else None)
+        counts, wsum = histogram2d_device(rf[0], rf[2], a.mat.edges, a.mat.edges, weights=amp)
+        a.result = {"rf": rf if self.keep_outputs else None,
+                    "amp": amp if self.keep_outputs else None,
+                    "image": wsum if wsum is not None else counts,
+                    "path": a.out_path if self.keep_outputs else None}
+        a.done = torch.cuda.Event(enable_timing=True)
+        a.done.record(self.main)
+
+    # -- main loop ------------------------------------------------------------
+    def run(self, requests, *, realtime: bool = True, max_lookahead: int = 2):
+        """Serve ``requests`` (arrival times in seconds from the start).
+
+        Returns (responses by rid, stats dict).  With ``realtime=False`` every
+        request is treated as already arrived.
+        """
+        L = N.lib()
+        queue = deque(sorted(requests, key=lambda r: (r.arrival if realtime else 0.0, r.rid)))
+        if not realtime:
+            for r in queue:
+                r.arrival = 0.0
+        adm = Admission(self.budget, self.max_slots)
+        responses: dict[int, Response] = {}
+        active: list[_Active] = []
+        inflight = deque()
+        torch.cuda.synchronize(self.dev)
+        t_start = time.perf_counter()
+        start_ev = torch.cuda.Event(enable_timing=True)
+        start_ev.record(self.main)
+        total_steps = 0
+        while queue or active:
+            now = time.perf_counter() - t_start
+            admitted, rejected = adm.admit(queue, now)
+            for r in rejected:
+                responses[r.rid] = Response(r.rid, "rejected",
+                                            f"footprint {r.footprint_bytes()} B exceeds the HBM "
+                                            f"budget {self.budget} B", arrival=r.arrival)
+            for r in admitted:
+                try:
+                    active.append(self._setup(r))
+                except (ValueError, NotImplementedError, N.SlothError) as e:
+                    adm.release(r)
+                    responses[r.rid] = Response(r.rid, "error", str(e), arrival=r.arrival)
+            runnable = [a for a in active if not a.finished and a.ready.query()]
+            runnable = runnable[:N.SLT_MAX_JOBS]
+            progressed = False
+            if runnable:
+                n_rays = sum(a.mat.rays.shape[1] for a in runnable)
+                k = int(max(16, min(4096, self.batch_work / max(n_rays, 1))))
+                table = N.JobTable()
+                table.n_jobs = len(runnable)
+                spans = []
+                for i, a in enumerate(runnable):
+                    end = a.sched if a.cursor >= a.inbox_estimate else min(a.sched, a.cursor + k)
+                    table.jobs[i] = self._job(a, a.cursor, end)
+                    spans.append(end)
+                N.check(L.slt_job_table_prepare(ctypes.byref(table)))
+                with torch.cuda.stream(self.main):
+                    self._upload(table)
+                    if self.graph is not None:
+                        self.graph.replay()
+                    else:
+                        N.check(L.slt_advance(self.table_dev.data_ptr(), self.blocks,
+                                              N.stream_handle()))
+                    self.batches += 1
+                    ev = torch.cuda.Event()
+                    ev.record(self.main)
+                    inflight.append(ev)
+                    for a, end in zip(runnable, spans):
+                        a.cursor = end
+                        if end >= a.sched:
+                            a.finished = True
+                            self._finalize(a)
+                progressed = True
+            for a in [a for a in active if a.finished and a.done is not None and a.done.query()]:
+                steps = int(a.counter.item())
+                total_steps += steps
+                r = a.req
+                responses[r.rid] = Response(
+                    r.rid, "ok", rf=a.result["rf"], amp=a.result["amp"],
+                    image=a.result["image"], path=a.result["path"], ray_steps=steps,
+                    arrival=r.arrival, completed=start_ev.elapsed_time(a.done) / 1e3,
+                    n_rays=a.mat.rays.shape[1])
+                active.remove(a)
+                adm.release(r)
+                progressed = True
+            while len(inflight) > max_lookahead:
+                inflight.popleft().synchronize()
+            if not progressed:
+                if active:
+                    time.sleep(20e-6)
+                elif queue:
+                    wait = queue[0].arrival - (time.perf_counter() - t_start)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+        torch.cuda.synchronize(self.dev)
+        makespan = max([r.completed for r in responses.values() if r.status == "ok"] or [0.0])
+        lat = np.array([r.latency for r in responses.values() if r.status == "ok"])
+        stats = {
+            "requests": len(responses),
+            "ok": int(sum(r.status == "ok" for r in responses.values())),
+            "rejected": int(sum(r.status == "rejected" for r in responses.values())),
+            "ray_steps": total_steps,
+            "makespan_s": makespan,
+            "ray_steps_per_s": total_steps / makespan if makespan > 0 else 0.0,
+            "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else math.nan,
+            "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else math.nan,
+            "batches": self.batches,
+        }
+        return responses, stats
+
+
+def run_offline(requests, *, step_fraction: float = 0.5, device=None):
+    """The same requests one after another through the offline path (slt_trace).
+
+    Returns (results by rid, stats).
+    """
+    dev = device if device is not None else N.require_device()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    out = {}
+    total = 0
+    for req in requests:
+        mat = materialize(req, dev)
+        res = engine.trace(mat.grid, mat.rays, mat.save_times, step_fraction=step_fraction,
+                           save_all=False, want_path=True)
+        rf, amp = jones_device(res.states[0], mat.grid.probing_direction,
+                               res.tau[0] if res.tau is not None else None)
+        counts, wsum = histogram2d_device(rf[0], rf[2], mat.edges, mat.edges, weights=amp)
+        out[req.rid] = {"rf": rf, "amp": amp, "image": wsum if wsum is not None else counts,
+                        "path": res.path, "steps": res.total_steps, "state": res.states[0]}
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - t0
+    for v in out.values():
+        v["steps"] = int(v["steps"].item())
+        total += v["steps"]
+    return out, {"ray_steps": total, "wall_s": wall, "ray_steps_per_s": total / wall}

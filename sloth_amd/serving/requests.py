@@ -1,0 +1,133 @@
+"""Ray-trace request specifications and their on-device synthesis.
+
+A request names its density field (kind, grid side, box length, seed), its
+beam (shape, half-width, ray count, seed), the wavelength and the absorption
+settings.  ``materialize`` builds the packed grid and the rays directly in HBM
+(rocFFT through torch for the Kolmogorov field, torch's device generator for
+the beam), so the host never touches grid-sized data.  The same spec always
+produces the same inputs, which is what lets a request be compared alone and
+inside a serving mix.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from ..constants import C_LIGHT
+from ..core import engine
+
+
+@dataclass(frozen=True)
+class FieldSpec:
+    kind: str = "kolmogorov"      # "kolmogorov" | "sine"
+    side: int = 128               # grid points per axis
+    length: float = 10e-3         # box edge [m]
+    seed: int = 0
+    n0: float = 2e24              # mean density [m^-3]
+    frac: float = 0.3             # relative fluctuation (kolmogorov) / sine amplitude
+    l_min: float = 0.2e-3         # smallest scale of the kolmogorov band [m]
+    wavelength: float = 2e-3      # sine period [m]
+
+
+@dataclass(frozen=True)
+class BeamSpec:
+    n_rays: int = 10000
+    half_width: float = 3e-3      # square beam half-width [m]
+    seed: int = 0
+
+
+@dataclass
+class RequestSpec:
+    rid: int
+    field: FieldSpec
+    beam: BeamSpec
+    lwl: float = 532e-9
+    inv_brems: bool = True
+    Te: float = 20.0
+    Z: float = 1.0
+    coulomb_log: float | None = 10.0
+    bins: int = 256
+    arrival: float = 0.0          # seconds after the start of a run
+    extra: dict = field(default_factory=dict)
+
+    def horizon(self) -> float:
+        """The reference solve horizon sqrt(8) L / c (src/core/propagator.py:616)."""
+        return math.sqrt(8.0) * self.field.length / C_LIGHT
+
+    def footprint_bytes(self) -> int:
+        """HBM needed while the request is resident (grid, transients, rays, outputs)."""
+        n3 = self.field.side ** 3
+        return int(16 * n3 + 16 * n3 + 200 * self.beam.n_rays + 8 * self.bins ** 2 + (1 << 20))
+
+
+def _coords(spec: FieldSpec):
+    c = np.float32(np.linspace(-spec.length / 2, spec.length / 2, spec.side))
+    return c, c.copy(), c.copy()
+
+
+def field_on_device(spec: FieldSpec, device) -> torch.Tensor:
+    """(n, n, n) float32 electron density [m^-3] built on the device."""
+    n = spec.side
+    if spec.kind == "sine":
+        x = torch.from_numpy(_coords(spec)[0].astype(np.float64)).to(device)
+        prof = spec.n0 * (1.0 + spec.frac * torch.sin(2 * math.pi * x / spec.wavelength))
+        return prof.to(torch.float32)[:, None, None].expand(n, n, n).contiguous()
+    if spec.kind != "kolmogorov":
+        raise ValueError(f"unknown field kind {spec.kind!r}")
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(spec.seed))
+    noise = torch.randn((n, n, n), generator=gen, device=device, dtype=torch.float32)
+    F = torch.fft.rfftn(noise)
+    del noise
+    dx = spec.length / (n - 1)
+    kx = torch.fft.fftfreq(n, d=dx, device=device, dtype=torch.float32) * (2 * math.pi)
+    kz = torch.fft.rfftfreq(n, d=dx, device=device, dtype=torch.float32) * (2 * math.pi)
+    k = torch.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    kmin, kmax = 2 * math.pi / spec.length, 2 * math.pi / spec.l_min
+    amp = torch.where((k >= kmin) & (k <= kmax), torch.clamp(k, min=kmin) ** (-11.0 / 6.0),
+                      torch.zeros_like(k))
+    del k
+    F *= amp
+    del amp
+    g = torch.fft.irfftn(F, s=(n, n, n))
+    del F
+    g = (g - g.mean()) / g.std()
+    return torch.clamp(spec.n0 * (1.0 + spec.frac * g), min=0.0).to(torch.float32).contiguous()
+
+
+def beam_on_device(spec: BeamSpec, length: float, device) -> torch.Tensor:
+    """(6, N) collimated square beam entering the z = -L/2 face, built on the device."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(spec.seed) + 7919)
+    n = spec.n_rays
+    s0 = torch.zeros((6, n), dtype=torch.float32, device=device)
+    s0[0:2] = (2 * torch.rand((2, n), generator=gen, device=device) - 1) * spec.half_width
+    s0[2] = float(np.float32(-length / 2))
+    s0[5] = C_LIGHT
+    return s0
+
+
+@dataclass
+class Materialized:
+    """Device-resident inputs of one request."""
+    grid: engine.PackedGrid
+    rays: torch.Tensor
+    save_times: np.ndarray
+    edges: np.ndarray
+
+
+def materialize(req: RequestSpec, device) -> Materialized:
+    ne = field_on_device(req.field, device)
+    omega = 2 * math.pi * C_LIGHT / req.lwl
+    grid = engine.PackedGrid(ne, *_coords(req.field), omega, inv_brems=req.inv_brems,
+                             Te=req.Te if req.inv_brems else None,
+                             Z=req.Z if req.inv_brems else None,
+                             coulomb_log=req.coulomb_log, device=device)
+    del ne
+    rays = beam_on_device(req.beam, req.field.length, device)
+    L = req.field.length
+    return Materialized(grid, rays, np.array([0.0, req.horizon()]),
+                        np.linspace(-L / 2, L / 2, req.bins + 1))

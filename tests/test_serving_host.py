@@ -1,0 +1,91 @@
+"""Serving host logic without a GPU: admission and the job-table checks of the C ABI."""
+import ctypes
+from collections import deque
+
+import numpy as np
+import pytest
+
+from sloth_amd import _native as N
+from sloth_amd.serving.engine import Admission
+from sloth_amd.serving.requests import BeamSpec, FieldSpec, RequestSpec
+
+
+def _req(rid, side=64, rays=10_000, arrival=0.0):
+    return RequestSpec(rid, FieldSpec(side=side), BeamSpec(n_rays=rays), arrival=arrival)
+
+
+def test_admission_fifo_budget_slots_and_rejection():
+    a, b, c = _req(0, arrival=0.0), _req(1, arrival=0.5), _req(2, side=128, arrival=0.6)
+    budget = a.footprint_bytes() + c.footprint_bytes()
+    adm = Admission(budget, max_slots=4)
+    q = deque([a, b, c])
+    got, rej = adm.admit(q, now=0.1)          # only a has arrived
+    assert [r.rid for r in got] == [0] and not rej and [r.rid for r in q] == [1, 2]
+    got, rej = adm.admit(q, now=1.0)          # b fits beside a; c must wait for memory
+    assert [r.rid for r in got] == [1] and not rej and [r.rid for r in q] == [2]
+    adm.release(a)
+    got, rej = adm.admit(q, now=1.0)
+    assert [r.rid for r in got] == [2] and not rej and not q
+    assert adm.used == b.footprint_bytes() + c.footprint_bytes() and adm.slots == 2
+
+
+def test_admission_rejects_oversized_and_keeps_going():
+    big = _req(0, side=384, rays=4_000_000)
+    small = _req(1)
+    adm = Admission(small.footprint_bytes() * 2, max_slots=1)
+    q = deque([big, small])
+    got, rej = adm.admit(q, now=0.0)
+    assert rej == [big] and got == [small]
+    q2 = deque([_req(2)])
+    assert adm.admit(q2, 0.0) == ([], [])  # slot limit
+    adm.release(small)
+    assert adm.admit(q2, 0.0)[0][0].rid == 2
+
+
+def _grid_desc(n=8):
+    c = np.float32(np.linspace(0, 1e-3, n))
+    d = N.GridDesc()
+    assert N.lib().slt_grid_desc_init(c.ctypes.data, c.ctypes.data, c.ctypes.data, n, n, n, 2, 1,
+                                      ctypes.byref(d)) == 0
+    return d
+
+
+def _valid_job(n_rays=1000):
+    j = N.Job()
+    j.packed = 0x1000
+    j.grid = _grid_desc()
+    j.state = 0x2000
+    j.n_rays = n_rays
+    j.seg_nsub = 0x3000
+    j.seg_h = 0x4000
+    j.n_seg = 1
+    j.sched_steps = 100
+    j.step_begin = 0
+    j.step_end = 50
+    return j
+
+
+def test_job_table_prepare_units_and_errors():
+    L = N.lib()
+    t = N.JobTable()
+    t.n_jobs = 3
+    for i, n in enumerate((1000, 256, 1)):
+        t.jobs[i] = _valid_job(n)
+    assert L.slt_job_table_prepare(ctypes.byref(t)) == N.SLT_OK
+    assert list(t.unit_begin[:4]) == [0, 4, 5, 6] and t.n_units == 6
+    # a finishing batch without an output buffer is refused
+    t.jobs[1].step_end = 100
+    assert L.slt_job_table_prepare(ctypes.byref(t)) == N.SLT_ERR_INVALID
+    assert "job 1" in N.last_error() and "out_state" in N.last_error()
+    t.jobs[1].step_end = 50
+    # a step range past the schedule is refused
+    t.jobs[2].step_end = 101
+    assert L.slt_job_table_prepare(ctypes.byref(t)) == N.SLT_ERR_INVALID
+    t.jobs[2].step_end = 10
+    # a grid with a plane stride beyond 24 bits is unsupported by the kernel
+    t.jobs[0].grid.stride[0] = 1 << 24
+    assert L.slt_job_table_prepare(ctypes.byref(t)) == N.SLT_ERR_UNSUPPORTED
+    t.n_jobs = N.SLT_MAX_JOBS + 1
+    assert L.slt_job_table_prepare(ctypes.byref(t)) == N.SLT_ERR_INVALID
+    with pytest.raises(ValueError):
+        N.check(N.SLT_ERR_INVALID)
