@@ -237,6 +237,57 @@ int slt_job_table_prepare(slt_job_table* host_table);
  */
 int slt_advance(const slt_job_table* device_table, int32_t blocks, slt_stream_t stream);
 
+/* ------------------------------------------------------------------------- */
+/* Adjoint: d(loss on final ray states)/d(n_e).  No reference counterpart     */
+/* (the reference has no gradient path; jax.grad through diffrax would be).  */
+/* ------------------------------------------------------------------------- */
+
+typedef struct slt_adjoint_args {
+    int64_t n_rays;
+    const float* s0;              /* 6 x n_rays start states (caller order)              */
+    const int32_t* order;         /* processing order (Morton) or NULL                    */
+    int32_t n_seg;                /* save schedule, as in slt_trace_args                  */
+    int32_t ckpt_every;           /* K: checkpoint every K schedule steps (e.g. 16)       */
+    const int32_t* seg_nsub;
+    const float* seg_h;
+    int64_t sched_steps;          /* sum of seg_nsub                                      */
+    void* checkpoints;            /* slt_checkpoint_bytes(n_rays, sched_steps, K) bytes   */
+    float* final_state;           /* forward out: 6 x n_rays final states (caller order)  */
+    double* final_tau;            /* forward out: n_rays final tau, or NULL               */
+    const float* seed_state;      /* backward in: d loss / d final state, 6 x n_rays      */
+    const double* seed_tau;       /* backward in: d loss / d final tau, or NULL           */
+    float* grid_adj;              /* backward out (accumulated): d loss / d voxel, packed layout, 4 floats per voxel */
+    void* tape;                   /* backward scratch: slt_tape_bytes(n_rays, K) bytes    */
+    unsigned long long* total_steps; /* forward: += in-box steps, or NULL                 */
+} slt_adjoint_args;
+
+int64_t slt_checkpoint_bytes(int64_t n_rays, int64_t sched_steps, int32_t ckpt_every);
+int64_t slt_tape_bytes(int64_t n_rays, int32_t ckpt_every);
+
+/* Forward trace (as slt_trace) that also stores every ray's state each K steps. */
+int slt_forward_ckpt(const void* packed, const slt_grid_desc* g, const slt_adjoint_args* a,
+                     slt_stream_t stream);
+
+/*
+ * Reverse pass: from the seeds (adjoints of the final states and tau), replay
+ * each K-step interval from its checkpoint and run the transposed RK4 steps,
+ * accumulating d loss / d voxel into grid_adj (zero it first; float atomics).
+ */
+int slt_adjoint(const void* packed, const slt_grid_desc* g, const slt_adjoint_args* a,
+                slt_stream_t stream);
+
+/*
+ * d loss / d n_e from the voxel adjoint: the transposed numpy.gradient stencil
+ * (src/core/propagator.py:90-92) times grad_coeff, plus d kappa / d n_e
+ * (propagator.py:147-164) times the kappa adjoint.  Arguments as in
+ * slt_grid_pack; dne is (nx, ny, nz) C-order fp32.
+ */
+int slt_adjoint_unpack(const float* grid_adj, const slt_grid_desc* g, const float* ne,
+                       const float* cx, const float* cy, const float* cz, double grad_coeff,
+                       const float* te, int64_t n_te, const float* zz, int64_t n_z, double omega,
+                       double e_charge, double epsilon_0, double speed_of_light,
+                       double coulomb_log, float* dne, slt_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

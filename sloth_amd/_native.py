@@ -21,6 +21,8 @@ EXPORTED = (
     "slt_version", "slt_last_error", "slt_grid_desc_init", "slt_grid_bytes", "slt_grid_pack",
     "slt_kappa", "slt_trace", "slt_jones", "slt_histogram2d", "slt_morton_keys",
     "slt_ray_state_bytes", "slt_ray_state_init", "slt_job_table_prepare", "slt_advance",
+    "slt_checkpoint_bytes", "slt_tape_bytes", "slt_forward_ckpt", "slt_adjoint",
+    "slt_adjoint_unpack",
 )
 
 SLT_MAX_JOBS = 64
@@ -51,6 +53,27 @@ class TraceArgs(ctypes.Structure):
         ("save_tau", ctypes.c_void_p),
         ("path", ctypes.c_void_p),
         ("steps", ctypes.c_void_p),
+        ("total_steps", ctypes.c_void_p),
+    ]
+
+
+class AdjointArgs(ctypes.Structure):
+    _fields_ = [
+        ("n_rays", ctypes.c_int64),
+        ("s0", ctypes.c_void_p),
+        ("order", ctypes.c_void_p),
+        ("n_seg", ctypes.c_int32),
+        ("ckpt_every", ctypes.c_int32),
+        ("seg_nsub", ctypes.c_void_p),
+        ("seg_h", ctypes.c_void_p),
+        ("sched_steps", ctypes.c_int64),
+        ("checkpoints", ctypes.c_void_p),
+        ("final_state", ctypes.c_void_p),
+        ("final_tau", ctypes.c_void_p),
+        ("seed_state", ctypes.c_void_p),
+        ("seed_tau", ctypes.c_void_p),
+        ("grid_adj", ctypes.c_void_p),
+        ("tape", ctypes.c_void_p),
         ("total_steps", ctypes.c_void_p),
     ]
 
@@ -121,6 +144,14 @@ def lib():
     L.slt_ray_state_init.argtypes = [vp, i64, vp, vp, vp]
     L.slt_job_table_prepare.argtypes = [ctypes.POINTER(JobTable)]
     L.slt_advance.argtypes = [vp, i32, vp]
+    L.slt_checkpoint_bytes.argtypes = [i64, i64, i32]
+    L.slt_checkpoint_bytes.restype = i64
+    L.slt_tape_bytes.argtypes = [i64, i32]
+    L.slt_tape_bytes.restype = i64
+    L.slt_forward_ckpt.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(AdjointArgs), vp]
+    L.slt_adjoint.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(AdjointArgs), vp]
+    L.slt_adjoint_unpack.argtypes = [vp, ctypes.POINTER(GridDesc), vp, vp, vp, vp, dbl, vp, i64,
+                                     vp, i64, dbl, dbl, dbl, dbl, dbl, vp, vp]
     for name in EXPORTED:
         getattr(L, name).restype = getattr(L, name).restype or ctypes.c_int
     _lib = L

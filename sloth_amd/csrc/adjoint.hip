@@ -1,0 +1,550 @@
+// k4: adjoint of the RK4 trace, for d(loss on the final ray states)/d(n_e).
+//
+// Discrete adjoint: the gradient of exactly the fixed-step RK4 map the
+// forward kernel integrates (same schedule, same straight-line moves), so a
+// central finite difference of the forward loss converges to it.
+//
+//  * slt_forward_ckpt runs the forward integration and stores each ray's state
+//    every K steps (a checkpoint), plus the final state and tau.
+//  * slt_adjoint walks the checkpoint intervals from last to first: it replays
+//    the interval from its checkpoint onto a small per-ray tape (position,
+//    velocity and step length of every move), then runs the transposed RK4
+//    step over the tape in reverse.  Each stage re-evaluates the cached cell's
+//    trilinear polynomial; its Jacobian gives the state adjoint, and the
+//    stage's share of the grid adjoint is accumulated per lane as the 8
+//    polynomial moments of the current cell and flushed to global memory
+//    (atomics, one float4 per corner) only when the cell changes.
+//  * slt_adjoint_unpack turns the voxel adjoint (d/dx, d/dy, d/dz of coeff ne,
+//    kappa) into dL/dn_e: the transposed numpy.gradient stencil times coeff,
+//    plus d kappa / d n_e times the kappa adjoint.
+#include <math.h>
+
+#include "integrate.h"
+
+namespace slt {
+
+// ---------------------------------------------------------------------------
+// Checkpoints: per checkpoint, SoA over rays:
+//   rb[3] f64 | tau f64 | off[3] f32 | dv[3] f32 | fold u32      (60 B per ray)
+// ---------------------------------------------------------------------------
+constexpr int64_t kCkptBytesPerRay = 4 * 8 + 6 * 4 + 4;
+
+struct CkptView {
+    double* rb[3];
+    double* tau;
+    float* off[3];
+    float* dv[3];
+    uint32_t* fold;
+};
+
+__device__ __forceinline__ CkptView ckpt_view(void* base, int64_t n, int c) {
+    CkptView v;
+    char* p = static_cast<char*>(base) + (int64_t)c * kCkptBytesPerRay * n;
+    double* d = reinterpret_cast<double*>(p);
+    for (int a = 0; a < 3; ++a) v.rb[a] = d + a * n;
+    v.tau = d + 3 * n;
+    float* f = reinterpret_cast<float*>(d + 4 * n);
+    for (int a = 0; a < 3; ++a) {
+        v.off[a] = f + a * n;
+        v.dv[a] = f + (3 + a) * n;
+    }
+    v.fold = reinterpret_cast<uint32_t*>(f + 6 * n);
+    return v;
+}
+
+// Tape: per slot (one move), SoA over rays: rh[3], vh[3], hT (h > 0 for an RK4
+// step, -T for a straight move of duration T).
+struct GlobalTape {
+    float* base;
+    int64_t n;
+    int64_t t;
+    int count;
+    __device__ __forceinline__ float* slot(int j) const { return base + (int64_t)j * 7 * n; }
+    __device__ __forceinline__ void rk(const float* rh, const float* vh, float h) {
+        float* p = slot(count++);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            p[c * n + t] = rh[c];
+            p[(3 + c) * n + t] = vh[c];
+        }
+        p[6 * n + t] = h;
+    }
+    __device__ __forceinline__ void straight(float T) {
+        float* p = slot(count++);
+        p[6 * n + t] = -T;
+    }
+};
+
+struct AdjArgs {
+    DevGrid G;
+    int64_t n;
+    const float* __restrict__ s0;
+    const int32_t* __restrict__ order;
+    int nseg;
+    int K;
+    int nck;
+    int64_t sched;
+    const int32_t* __restrict__ seg_nsub;
+    const float* __restrict__ seg_h;
+    void* ckpt;
+    float* __restrict__ final_state;
+    double* __restrict__ final_tau;
+    const float* __restrict__ seed_state;
+    const double* __restrict__ seed_tau;
+    float* __restrict__ grid_adj;
+    float* __restrict__ tape;
+    unsigned long long* __restrict__ total;
+    int nblocks;
+};
+
+template <bool KAPPA>
+__global__ void __launch_bounds__(256) forward_ckpt_kernel(AdjArgs A) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x, A.nblocks);
+    if (lb < 0) return;
+    const int64_t t = (int64_t)lb * blockDim.x + threadIdx.x;
+    uint32_t nsteps = 0;
+    if (t < A.n) {
+        const int64_t ray = A.order ? (int64_t)A.order[t] : t;
+        RayState s;
+        state_from_rays(A.s0, A.n, ray, s);
+        SaveSink sink{A.final_state, A.final_tau, A.n, ray, 0, A.nseg};
+        for (int c = 0; c < A.nck; ++c) {
+            const CkptView v = ckpt_view(A.ckpt, A.n, c);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                v.rb[a][t] = s.rb[a];
+                v.off[a][t] = s.off[a];
+                v.dv[a][t] = s.dv[a];
+            }
+            v.tau[t] = s.tau;
+            v.fold[t] = s.fold;
+            const int64_t g0 = (int64_t)c * A.K;
+            const int64_t g1 = g0 + A.K < A.sched ? g0 + A.K : A.sched;
+            integrate<KAPPA, false>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1, sink);
+        }
+        nsteps = s.steps;
+    }
+    if (A.total) add_steps(A.total, nsteps);
+}
+
+// Per-lane grid-adjoint accumulator: the 8 polynomial moments (per channel) of
+// the stage adjoints that fell into one cell.
+struct Moments {
+    int id;
+    float4 m0, mx, my, mz, mxy, mxz, myz, mxyz;
+};
+
+__device__ __forceinline__ void f4fma(float4& acc, float s, const float4& a) {
+    acc.x = fmaf(s, a.x, acc.x);
+    acc.y = fmaf(s, a.y, acc.y);
+    acc.z = fmaf(s, a.z, acc.z);
+    acc.w = fmaf(s, a.w, acc.w);
+}
+
+__device__ __forceinline__ void f4atomic(float* p, float4 v) {
+    atomicAdd(p + 0, v.x);
+    atomicAdd(p + 1, v.y);
+    atomicAdd(p + 2, v.z);
+    atomicAdd(p + 3, v.w);
+}
+
+__device__ __forceinline__ float4 f4add(float4 a, float4 b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+// Corner adjoints of the accumulated moments: corner (ex, ey, ez) has weight
+// prod_a (e_a ? t_a : 1 - t_a), whose monomial expansion gives these sums.
+__device__ __forceinline__ void flush(const DevGrid& G, float* __restrict__ adj, Moments& M) {
+    if (M.id < 0) return;
+    const float4 c111 = M.mxyz;
+    const float4 c110 = f4sub(M.mxy, M.mxyz);
+    const float4 c101 = f4sub(M.mxz, M.mxyz);
+    const float4 c011 = f4sub(M.myz, M.mxyz);
+    const float4 c100 = f4sub(f4sub(M.mx, M.mxy), c101);
+    const float4 c010 = f4sub(f4sub(M.my, M.mxy), c011);
+    const float4 c001 = f4sub(f4sub(M.mz, M.mxz), c011);
+    const float4 c000 = f4sub(f4sub(f4sub(M.m0, M.mx), f4add(M.my, M.mz)),
+                              f4sub(f4sub(f4sub(M.mxyz, M.mxy), M.mxz), M.myz));
+    float* p = adj + (int64_t)M.id * 4;
+    f4atomic(p, c000);
+    f4atomic(p + 4 * G.s2, c001);
+    f4atomic(p + 4 * G.s1, c010);
+    f4atomic(p + 4 * (G.s1 + G.s2), c011);
+    f4atomic(p + 4 * G.s0, c100);
+    f4atomic(p + 4 * (G.s0 + G.s2), c101);
+    f4atomic(p + 4 * (G.s0 + G.s1), c110);
+    f4atomic(p + 4 * (G.s0 + G.s1 + G.s2), c111);
+    M.id = -1;
+}
+
+// Adjoint of one RHS evaluation at p with output adjoint Ab (gx, gy, gz, kappa):
+// accumulates the grid adjoint and returns d(loss)/d(p) in rb.
+__device__ __forceinline__ void eval_adj(const DevGrid& G, float x, float y, float z,
+                                         const float4& Ab, Cell& cc, Moments& M,
+                                         float* __restrict__ adj, float rb[3]) {
+    rb[0] = rb[1] = rb[2] = 0.f;
+    if (!in_box(G, x, y, z)) return;
+    const float u0 = (x - G.lo0) * G.id0;
+    const float u1 = (y - G.lo1) * G.id1;
+    const float u2 = (z - G.lo2) * G.id2;
+    const int i0 = min((int)u0, G.n0 - 2);
+    const int i1 = min((int)u1, G.n1 - 2);
+    const int i2 = min((int)u2, G.n2 - 2);
+    const float tx = u0 - (float)i0;
+    const float ty = u1 - (float)i1;
+    const float tz = u2 - (float)i2;
+    const int id = (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2));
+    if (id != cc.id) load_cell(G, id, cc);
+    // channel-contracted coefficients: Kq = sum_ch Ab_ch * kq_ch
+    auto dot = [&](const float4& k) {
+        return fmaf(Ab.w, k.w, fmaf(Ab.z, k.z, fmaf(Ab.y, k.y, Ab.x * k.x)));
+    };
+    const float Kx = dot(cc.kx), Ky = dot(cc.ky), Kz = dot(cc.kz);
+    const float Kxy = dot(cc.kxy), Kxz = dot(cc.kxz), Kyz = dot(cc.kyz), Kxyz = dot(cc.kxyz);
+    // d f / d t_a of f = k0 + kx tx + ky ty + kz tz + kxy tx ty + kxz tx tz + kyz ty tz + kxyz tx ty tz
+    rb[0] = G.id0 * fmaf(tz, fmaf(ty, Kxyz, Kxz), fmaf(ty, Kxy, Kx));
+    rb[1] = G.id1 * fmaf(tz, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxy, Ky));
+    rb[2] = G.id2 * fmaf(ty, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxz, Kz));
+    if (id != M.id) {
+        flush(G, adj, M);
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = z4;
+        M.id = id;
+    }
+    const float txy = tx * ty, txz = tx * tz, tyz = ty * tz, txyz = txy * tz;
+    M.m0 = f4add(M.m0, Ab);
+    f4fma(M.mx, tx, Ab);
+    f4fma(M.my, ty, Ab);
+    f4fma(M.mz, tz, Ab);
+    f4fma(M.mxy, txy, Ab);
+    f4fma(M.mxz, txz, Ab);
+    f4fma(M.myz, tyz, Ab);
+    f4fma(M.mxyz, txyz, Ab);
+}
+
+// Transposed RK4 step.  On entry rb, vb are the adjoints of the state after the
+// step; on exit, of the state before it.  tb (tau adjoint) is unchanged by a step.
+template <bool KAPPA>
+__device__ __forceinline__ void step_adj(const DevGrid& G, const float rh[3], const float vh[3],
+                                         float h, float rb[3], float vb[3], float tb, Cell& cc,
+                                         Moments& M, float* __restrict__ adj) {
+    const float hh = 0.5f * h;
+    const float h6 = h * (1.0f / 6.0f);
+    const float h3 = h * (1.0f / 3.0f);
+    const float hh6 = h * h6;
+    // forward stage points (the same arithmetic as integrate())
+    float4 A1 = make_float4(0.f, 0.f, 0.f, 0.f), A2;
+    if (in_box(G, rh[0], rh[1], rh[2])) eval_rhs<KAPPA, false>(G, rh[0], rh[1], rh[2], cc, A1);
+    const float r2[3] = {fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]), fmaf(hh, vh[2], rh[2])};
+    const float v2[3] = {fmaf(hh, A1.x, vh[0]), fmaf(hh, A1.y, vh[1]), fmaf(hh, A1.z, vh[2])};
+    eval_rhs<KAPPA, true>(G, r2[0], r2[1], r2[2], cc, A2);
+    const float r3[3] = {fmaf(hh, v2[0], rh[0]), fmaf(hh, v2[1], rh[1]), fmaf(hh, v2[2], rh[2])};
+    const float v3[3] = {fmaf(hh, A2.x, vh[0]), fmaf(hh, A2.y, vh[1]), fmaf(hh, A2.z, vh[2])};
+    const float r4[3] = {fmaf(h, v3[0], rh[0]), fmaf(h, v3[1], rh[1]), fmaf(h, v3[2], rh[2])};
+    const float tk = KAPPA ? tb : 0.f;
+    float p4[3], p3[3], p2[3], p1[3];
+    // stage 4: a4 enters v' with weight h/6
+    const float4 B4 = make_float4(h6 * vb[0], h6 * vb[1], h6 * vb[2], h6 * tk);
+    eval_adj(G, r4[0], r4[1], r4[2], B4, cc, M, adj, p4);
+    const float v3b[3] = {h * p4[0], h * p4[1], h * p4[2]};  // r4 = r + h v3
+    // stage 3: a3 enters r' (h^2/6) and v' (h/3)
+    const float4 B3 = make_float4(fmaf(hh6, rb[0], h3 * vb[0]), fmaf(hh6, rb[1], h3 * vb[1]),
+                                  fmaf(hh6, rb[2], h3 * vb[2]), h3 * tk);
+    eval_adj(G, r3[0], r3[1], r3[2], B3, cc, M, adj, p3);
+    const float v2b[3] = {hh * p3[0], hh * p3[1], hh * p3[2]};  // r3 = r + h/2 v2
+    // stage 2: a2 enters r' (h^2/6), v' (h/3) and v3 = v + h/2 a2
+    const float4 B2 = make_float4(fmaf(hh, v3b[0], fmaf(hh6, rb[0], h3 * vb[0])),
+                                  fmaf(hh, v3b[1], fmaf(hh6, rb[1], h3 * vb[1])),
+                                  fmaf(hh, v3b[2], fmaf(hh6, rb[2], h3 * vb[2])), h3 * tk);
+    eval_adj(G, r2[0], r2[1], r2[2], B2, cc, M, adj, p2);
+    // stage 1: a1 enters r' (h^2/6), v' (h/6) and v2 = v + h/2 a1
+    const float4 B1 = make_float4(fmaf(hh, v2b[0], fmaf(hh6, rb[0], h6 * vb[0])),
+                                  fmaf(hh, v2b[1], fmaf(hh6, rb[1], h6 * vb[1])),
+                                  fmaf(hh, v2b[2], fmaf(hh6, rb[2], h6 * vb[2])), h6 * tk);
+    eval_adj(G, rh[0], rh[1], rh[2], B1, cc, M, adj, p1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        // v: r2 = r + h/2 v, v2, v3, r' = r + h v + ..., v' = v + ...
+        const float nv = vb[c] + h * rb[c] + v3b[c] + v2b[c] + hh * p2[c];
+        rb[c] = rb[c] + p1[c] + p2[c] + p3[c] + p4[c];
+        vb[c] = nv;
+    }
+}
+
+template <bool KAPPA>
+__global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x, A.nblocks);
+    if (lb < 0) return;
+    const int64_t t = (int64_t)lb * blockDim.x + threadIdx.x;
+    if (t >= A.n) return;
+    const int64_t n = A.n;
+    const int64_t ray = A.order ? (int64_t)A.order[t] : t;
+    float rb[3], vb[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        rb[c] = A.seed_state[c * n + ray];
+        vb[c] = A.seed_state[(3 + c) * n + ray];
+    }
+    const float tb = A.seed_tau ? (float)A.seed_tau[ray] : 0.f;
+    float v0[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v0[c] = A.s0[(3 + c) * n + ray];
+    Cell cc;
+    cc.id = -1;
+    Moments M;
+    M.id = -1;
+    const SaveSink none{nullptr, nullptr, n, ray, 0, A.nseg};
+    for (int c = A.nck - 1; c >= 0; --c) {
+        const CkptView v = ckpt_view(A.ckpt, n, c);
+        RayState s;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            s.rb[a] = v.rb[a][t];
+            s.off[a] = v.off[a][t];
+            s.dv[a] = v.dv[a][t];
+            s.v0[a] = v0[a];
+        }
+        s.tau = v.tau[t];
+        s.fold = v.fold[t];
+        s.path = 0.0;
+        s.steps = 0;
+        const int64_t g0 = (int64_t)c * A.K;
+        const int64_t g1 = g0 + A.K < A.sched ? g0 + A.K : A.sched;
+        GlobalTape tape{A.tape, n, t, 0};
+        integrate<KAPPA, false, GlobalTape>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1, none,
+                                            tape);
+        for (int j = tape.count - 1; j >= 0; --j) {
+            const float* p = tape.slot(j);
+            const float hT = p[6 * n + t];
+            if (hT < 0.f) {
+                // straight move r' = r + T v
+#pragma unroll
+                for (int a = 0; a < 3; ++a) vb[a] = fmaf(-hT, rb[a], vb[a]);
+            } else {
+                float rh[3], vh[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    rh[a] = p[a * n + t];
+                    vh[a] = p[(3 + a) * n + t];
+                }
+                step_adj<KAPPA>(A.G, rh, vh, hT, rb, vb, tb, cc, M, A.grid_adj);
+            }
+        }
+    }
+    flush(A.G, A.grid_adj, M);
+}
+
+// ---------------------------------------------------------------------------
+// Voxel adjoint -> dL/dn_e
+// ---------------------------------------------------------------------------
+struct UnpackArgs {
+    const float* __restrict__ adj;  // packed layout, 4 floats per voxel
+    const float* __restrict__ ne;
+    const float* __restrict__ c0;
+    const float* __restrict__ c1;
+    const float* __restrict__ c2;
+    double coeff;
+    const float* __restrict__ te;
+    int64_t n_te;
+    const float* __restrict__ zz;
+    int64_t n_z;
+    double omega, e_charge, epsilon_0, c_light, coulomb_log;
+    int n0, n1, n2;
+    int64_t s0, s1, s2;
+    float* __restrict__ dne;
+};
+
+// Coefficient of f[j] in the gradient output P[i] along an axis of n points
+// (numpy.gradient with coordinates, edge_order=1).
+__device__ inline double grad_coef(const float* __restrict__ c, int n, int i, int j) {
+    if (i == 0) {
+        const double d = (double)c[1] - (double)c[0];
+        return j == 0 ? -1.0 / d : (j == 1 ? 1.0 / d : 0.0);
+    }
+    if (i == n - 1) {
+        const double d = (double)c[n - 1] - (double)c[n - 2];
+        return j == n - 1 ? 1.0 / d : (j == n - 2 ? -1.0 / d : 0.0);
+    }
+    const double dx1 = (double)c[i] - (double)c[i - 1];
+    const double dx2 = (double)c[i + 1] - (double)c[i];
+    if (j == i - 1) return -dx2 / (dx1 * (dx1 + dx2));
+    if (j == i) return (dx2 - dx1) / (dx1 * dx2);
+    if (j == i + 1) return dx1 / (dx2 * (dx1 + dx2));
+    return 0.0;
+}
+
+// d kappa / d n_e of the NRL rate (src/core/propagator.py:147-164); the
+// Coulomb logarithm depends on n_e only through omega_pe when omega_pe > omega
+// and it is neither clamped nor overridden.
+__device__ inline double dkappa_dne(double ne, double te, double z, const UnpackArgs& A) {
+    if (ne <= 0.0) return 0.0;
+    const double pi = 3.14159265358979311600;
+    const double ne_cc = ne * 1e-6;
+    const double v_the = 4.19e5 * sqrt(te);
+    const double o_pe = 5.64e4 * sqrt(ne_cc);
+    const double o_max = o_pe > A.omega ? o_pe : A.omega;
+    const double b_min = z * A.e_charge / (4.0 * pi * A.epsilon_0 * te);
+    double cl = log(v_the / (o_max * b_min));
+    double dcl = o_pe > A.omega ? -0.5 / ne : 0.0;
+    if (!(cl >= 2.0)) { cl = 2.0; dcl = 0.0; }
+    if (A.coulomb_log > 0.0) { cl = A.coulomb_log; dcl = 0.0; }
+    const double r = ne_cc / A.omega;
+    const double kap = 3.1e-5 * z * A.c_light * (r * r) * cl * pow(te, -1.5);
+    return kap * (2.0 / ne + dcl / cl);
+}
+
+__global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs A) {
+    const int64_t total = (int64_t)A.n0 * A.n1 * A.n2;
+    const int64_t sy = A.n2, sx = (int64_t)A.n1 * A.n2;
+    for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < total;
+         lin += (int64_t)gridDim.x * blockDim.x) {
+        const int idx[3] = {(int)(lin / sx), (int)((lin % sx) / sy), (int)(lin % sy)};
+        const int64_t pk = idx[0] * A.s0 + idx[1] * A.s1 + idx[2] * A.s2;
+        const int nn[3] = {A.n0, A.n1, A.n2};
+        const int64_t ps[3] = {A.s0, A.s1, A.s2};
+        const float* cs[3] = {A.c0, A.c1, A.c2};
+        double f = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const int j = idx[a];
+            for (int i = j - 1; i <= j + 1; ++i) {
+                if (i < 0 || i >= nn[a]) continue;
+                const double w = grad_coef(cs[a], nn[a], i, j);
+                if (w != 0.0) f += w * (double)A.adj[(pk + (int64_t)(i - j) * ps[a]) * 4 + a];
+            }
+        }
+        double d = A.coeff * f;
+        if (A.te) {
+            const double te = (double)A.te[A.n_te == 1 ? 0 : lin];
+            const double z = (double)A.zz[A.n_z == 1 ? 0 : lin];
+            d += dkappa_dne((double)A.ne[lin], te, z, A) * (double)A.adj[pk * 4 + 3];
+        }
+        A.dne[lin] = (float)d;
+    }
+}
+
+static int fill_adj_args(const void* packed, const slt_grid_desc* g, const slt_adjoint_args* a,
+                         AdjArgs* A, const char* who) {
+    int rc = make_dev_grid(packed, g, &A->G);
+    if (rc) return rc;
+    if (!a || a->n_rays < 0 || a->n_seg < 1 || a->ckpt_every < 1 || !a->seg_nsub || !a->seg_h ||
+        (a->n_rays > 0 && (!a->s0 || !a->checkpoints))) {
+        set_error("%s: invalid arguments", who);
+        return SLT_ERR_INVALID;
+    }
+    A->n = a->n_rays;
+    A->s0 = a->s0;
+    A->order = a->order;
+    A->nseg = a->n_seg;
+    A->K = a->ckpt_every;
+    A->sched = a->sched_steps;
+    A->nck = (int)((a->sched_steps + a->ckpt_every - 1) / a->ckpt_every);
+    if (A->nck < 1) A->nck = 1;
+    A->seg_nsub = a->seg_nsub;
+    A->seg_h = a->seg_h;
+    A->ckpt = a->checkpoints;
+    A->final_state = a->final_state;
+    A->final_tau = a->final_tau;
+    A->seed_state = a->seed_state;
+    A->seed_tau = a->seed_tau;
+    A->grid_adj = a->grid_adj;
+    A->tape = static_cast<float*>(a->tape);
+    A->total = a->total_steps;
+    const int64_t nb = (a->n_rays + 255) / 256;
+    if (nb > INT32_MAX / 2) {
+        set_error("%s: too many rays", who);
+        return SLT_ERR_UNSUPPORTED;
+    }
+    A->nblocks = (int)nb;
+    return SLT_OK;
+}
+
+}  // namespace slt
+
+using namespace slt;
+
+extern "C" int64_t slt_checkpoint_bytes(int64_t n_rays, int64_t sched_steps, int32_t ckpt_every) {
+    if (n_rays < 0 || sched_steps < 0 || ckpt_every < 1) return 0;
+    int64_t nck = (sched_steps + ckpt_every - 1) / ckpt_every;
+    if (nck < 1) nck = 1;
+    return nck * kCkptBytesPerRay * n_rays;
+}
+
+extern "C" int64_t slt_tape_bytes(int64_t n_rays, int32_t ckpt_every) {
+    if (n_rays < 0 || ckpt_every < 1) return 0;
+    return (int64_t)ckpt_every * 7 * 4 * n_rays;
+}
+
+extern "C" int slt_forward_ckpt(const void* packed, const slt_grid_desc* g,
+                                const slt_adjoint_args* a, slt_stream_t stream) {
+    clear_error();
+    AdjArgs A;
+    int rc = fill_adj_args(packed, g, a, &A, "slt_forward_ckpt");
+    if (rc) return rc;
+    if (A.n == 0) return SLT_OK;
+    if (!A.final_state) {
+        set_error("slt_forward_ckpt: final_state is required");
+        return SLT_ERR_INVALID;
+    }
+    const int grid = xcd_pad(A.nblocks);
+    if (g->nchan >= 4)
+        hipLaunchKernelGGL(forward_ckpt_kernel<true>, dim3(grid), dim3(256), 0,
+                           (hipStream_t)stream, A);
+    else
+        hipLaunchKernelGGL(forward_ckpt_kernel<false>, dim3(grid), dim3(256), 0,
+                           (hipStream_t)stream, A);
+    return check_launch("slt_forward_ckpt");
+}
+
+extern "C" int slt_adjoint(const void* packed, const slt_grid_desc* g, const slt_adjoint_args* a,
+                           slt_stream_t stream) {
+    clear_error();
+    AdjArgs A;
+    int rc = fill_adj_args(packed, g, a, &A, "slt_adjoint");
+    if (rc) return rc;
+    if (A.n == 0) return SLT_OK;
+    if (!A.seed_state || !A.grid_adj || !A.tape) {
+        set_error("slt_adjoint: seed_state, grid_adj and tape are required");
+        return SLT_ERR_INVALID;
+    }
+    const int grid = xcd_pad(A.nblocks);
+    if (g->nchan >= 4)
+        hipLaunchKernelGGL(adjoint_kernel<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, A);
+    else
+        hipLaunchKernelGGL(adjoint_kernel<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream,
+                           A);
+    return check_launch("slt_adjoint");
+}
+
+extern "C" int slt_adjoint_unpack(const float* grid_adj, const slt_grid_desc* g, const float* ne,
+                                  const float* cx, const float* cy, const float* cz,
+                                  double grad_coeff, const float* te, int64_t n_te,
+                                  const float* zz, int64_t n_z, double omega, double e_charge,
+                                  double epsilon_0, double speed_of_light, double coulomb_log,
+                                  float* dne, slt_stream_t stream) {
+    clear_error();
+    if (!grid_adj || !g || !ne || !cx || !cy || !cz || !dne) {
+        set_error("slt_adjoint_unpack: null pointer");
+        return SLT_ERR_INVALID;
+    }
+    const int64_t total = (int64_t)g->n[0] * g->n[1] * g->n[2];
+    if ((te || zz) && (!te || !zz || (n_te != 1 && n_te != total) || (n_z != 1 && n_z != total))) {
+        set_error("slt_adjoint_unpack: Te/Z must both be given with 1 or nx*ny*nz elements");
+        return SLT_ERR_INVALID;
+    }
+    UnpackArgs A;
+    A.adj = grid_adj;
+    A.ne = ne;
+    A.c0 = cx; A.c1 = cy; A.c2 = cz;
+    A.coeff = grad_coeff;
+    A.te = te; A.n_te = n_te;
+    A.zz = zz; A.n_z = n_z;
+    A.omega = omega; A.e_charge = e_charge; A.epsilon_0 = epsilon_0; A.c_light = speed_of_light;
+    A.coulomb_log = coulomb_log;
+    A.n0 = g->n[0]; A.n1 = g->n[1]; A.n2 = g->n[2];
+    A.s0 = g->stride[0]; A.s1 = g->stride[1]; A.s2 = g->stride[2];
+    A.dne = dne;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, A);
+    return check_launch("slt_adjoint_unpack");
+}
