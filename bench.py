@@ -155,6 +155,49 @@ def serving_bench(n_requests):
     }
 
 
+def adjoint_bench(n_rays=2_000_000, n=256):
+    """Config A: 256^3 Kolmogorov (10 mm), 2e6 rays at 532 nm with absorption; loss = MSE of
+    detector positions and energies against a trace through a second seeded grid;
+    checkpoints every 16 steps.  Times the checkpointed forward and the backward."""
+    import torch
+
+    from sloth_amd.adjoint import detector_loss, trace_states
+    from sloth_amd.fields import kolmogorov_field
+    from sloth_amd.core.beam import Beam
+
+    dev = torch.device("cuda", 0)
+    L = 10e-3
+    ne, coords = kolmogorov_field(n, L, n0=2e24, frac=0.3, l_min=0.2e-3, seed=21, device=dev)
+    tne, _ = kolmogorov_field(n, L, n0=2e24, frac=0.3, l_min=0.2e-3, seed=22, device=dev)
+    s0 = torch.from_numpy(Beam(n_rays, 4e-3, 0.0, L / 2, beam_type="square",
+                               seeded=True).s0).to(dev)
+    kw = dict(lwl=532e-9, inv_brems=True, Te=20.0, Z=1.0, coulomb_log=10.0, ckpt_every=16)
+    with torch.no_grad():
+        ts, tt, _ = trace_states(torch.from_numpy(tne).to(dev), s0, *coords, **kw)
+    tp, te = ts[0:2].double(), torch.exp(-tt)
+    ne_t = torch.from_numpy(ne).to(dev)
+    times = []
+    for rep in range(3):
+        x = ne_t.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, tau, total = trace_states(x, s0, *coords, **kw)
+        loss = detector_loss(st, tau, tp, te, pos_scale=1e-4)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        times.append((t1 - t0, t2 - t1))
+    fwd, bwd = min(t[0] for t in times), min(t[1] for t in times)
+    steps = int(total.item())
+    log(f"[adjoint] forward {fwd * 1e3:.1f} ms, backward {bwd * 1e3:.1f} ms, {steps} ray-steps")
+    return {"workload": "kolmogorov_256_2e6_rays_absorption", "ray_steps": steps,
+            "forward_ckpt_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
+            "backward_ray_steps_per_s": steps / bwd, "ckpt_every": 16,
+            "grad_norm": float(x.grad.double().norm())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +207,7 @@ def main():
     ap.add_argument("--rays", type=int, default=10_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=48,
                     help="requests in the serving mix (0 skips the serving leg)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -262,6 +306,11 @@ def main():
         log("[bench] serving mix ...")
         torch.cuda.empty_cache()
         serving = serving_bench(args.serving_requests)
+    adjoint = None
+    if not args.no_adjoint:
+        log("[bench] adjoint ...")
+        torch.cuda.empty_cache()
+        adjoint = adjoint_bench()
     out = {
         "metric": "ray-steps/s (512^3 grid, 1e7 rays)",
         "value": value,
@@ -283,6 +332,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
+        "adjoint": adjoint,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]

@@ -165,3 +165,82 @@ def test_adjoint_independent_of_checkpoint_interval():
         grads.append(ne_t.grad.double().cpu().numpy())
     for g in grads[1:]:
         assert np.linalg.norm(g - grads[0]) <= 1e-5 * np.linalg.norm(grads[0])
+
+
+def _line_update(orc, ne64, c64, grads, kap, idx, omega, cl):
+    """Recompute, in place, the gradient entries and kappa that depend on ne64[idx]."""
+    coeff = orc.gradient_coeff(omega)
+    for a in range(3):
+        sl = list(idx)
+        sl[a] = slice(None)
+        line = ne64[tuple(sl)] * coeff
+        grads[a][tuple(sl)] = np.gradient(line, c64[a], edge_order=1)
+    kap[idx] = orc.kappa(np.array([ne64[idx]]), 20.0, 1.0, omega, coulomb_log=cl)[0]
+
+
+def test_adjoint_full_size_grid_subset_fd(orc):
+    """Config-A grid (256^3 Kolmogorov, 10 mm, 532 nm, absorption): dL/dn_e of the loss of a
+    random 4000-ray subset of the 2e6-ray beam against oracle central differences."""
+    import torch
+
+    from sloth_amd.adjoint import detector_loss, trace_states
+    from sloth_amd.core import engine
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import kolmogorov_field
+
+    L, n, lwl, cl = 10e-3, 256, 532e-9, 10.0
+    ne, coords = kolmogorov_field(n, L, seed=11, n0=2e24, frac=0.3, l_min=0.2e-3)
+    tne, _ = kolmogorov_field(n, L, seed=12, n0=2e24, frac=0.3, l_min=0.2e-3)
+    s_all = Beam(2_000_000, 4e-3, 0.0, L / 2, beam_type="square", seeded=True).s0
+    s0 = s_all[:, np.random.default_rng(3).choice(s_all.shape[1], 4000, replace=False)]
+    omega = 2 * np.pi * C / lwl
+    c64 = [c.astype(np.float64) for c in coords]
+    s64 = s0.astype(np.float64)
+    g = engine.PackedGrid(ne, *coords, omega, inv_brems=True, Te=20.0, Z=1.0, coulomb_log=cl)
+    T = np.sqrt(8.0) * L / C
+    nsub, h = engine.schedule(np.array([0.0, T]), engine.nominal_step(g, 0.5))
+    h32 = h.astype(np.float32).astype(np.float64)
+    tg = orc.gradients(tne.astype(np.float64), *c64, omega)
+    tk = orc.kappa(tne.astype(np.float64), 20.0, 1.0, omega, coulomb_log=cl)
+    tsv, _ = orc.trace(tg, tk, c64, s64, nsub, h32)
+    tp, te_ = tsv[-1][0:2], np.exp(-tsv[-1][6])
+    del tg, tk
+    pos_scale = 1e-4
+
+    ne_t = torch.from_numpy(ne).cuda().requires_grad_(True)
+    state, tau, _ = trace_states(ne_t, s0, *coords, lwl=lwl, inv_brems=True, Te=20.0, Z=1.0,
+                                 coulomb_log=cl, save_times=np.array([0.0, T]))
+    detector_loss(state, tau, torch.from_numpy(tp).cuda(), torch.from_numpy(te_).cuda(),
+                  pos_scale=pos_scale).backward()
+    grad = ne_t.grad.double().cpu().numpy()
+
+    ne64 = ne.astype(np.float64)
+    grads = list(orc.gradients(ne64, *c64, omega))
+    kap = orc.kappa(ne64, 20.0, 1.0, omega, coulomb_log=cl)
+
+    def loss():
+        saves, _ = orc.trace(grads, kap, c64, s64, nsub, h32)
+        fin = saves[-1]
+        return float(np.mean(((fin[0:2] - tp) ** 2).sum(0) / pos_scale ** 2
+                             + (np.exp(-fin[6]) - te_) ** 2))
+
+    rng = np.random.default_rng(9)
+    cand = np.argwhere(np.abs(grad) > 0.05 * np.abs(grad).max())
+    pick = cand[rng.choice(len(cand), 64, replace=False)]
+    fd, ad = [], []
+    for idx in map(tuple, pick):
+        x = ne64[idx]
+        d = 1e-4 * x
+        vals = []
+        for v in (x + d, x - d):
+            ne64[idx] = v
+            _line_update(orc, ne64, c64, grads, kap, idx, omega, cl)
+            vals.append(loss())
+        ne64[idx] = x
+        _line_update(orc, ne64, c64, grads, kap, idx, omega, cl)
+        fd.append((vals[0] - vals[1]) / (2 * d))
+        ad.append(grad[idx])
+    fd, ad = np.array(fd), np.array(ad)
+    rel = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
+    print("full-size adjoint vs FD: normwise", rel, "worst", np.max(np.abs(ad - fd) / np.abs(fd)))
+    assert rel <= 2e-3
