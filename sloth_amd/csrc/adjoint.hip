@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(256) forward_ckpt_kernel(AdjArgs A) {
 // Per-lane grid-adjoint accumulator: the 8 polynomial moments (per channel) of
 // the stage adjoints that fell into one cell.
 struct Moments {
-    int id;
+    int id;  // cell (voxel index of corner (i, j, k)); -1 = empty
     float4 m0, mx, my, mz, mxy, mxz, myz, mxyz;
 };
 
@@ -141,6 +141,34 @@ __device__ __forceinline__ void f4fma(float4& acc, float s, const float4& a) {
     acc.w = fmaf(s, a.w, acc.w);
 }
 
+__device__ __forceinline__ float4 f4add(float4 a, float4 b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+// Wavefront sum of v, valid in lane 63: row-wise shifts (DPP row_shr 1, 2, 4, 8)
+// leave each 16-lane row's sum in its last lane, then row_bcast 15 / 31 fold the
+// rows into lane 63.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dppf(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                  CTRL, ROW_MASK, 0xf, false));
+}
+
+__device__ __forceinline__ float wave_sum63(float v) {
+    v += dppf<0x111, 0xf>(v);
+    v += dppf<0x112, 0xf>(v);
+    v += dppf<0x114, 0xf>(v);
+    v += dppf<0x118, 0xf>(v);
+    v += dppf<0x142, 0xa>(v);
+    v += dppf<0x143, 0xc>(v);
+    return v;
+}
+
+__device__ __forceinline__ float4 wave_sum63(const float4& v, bool mine) {
+    return make_float4(wave_sum63(mine ? v.x : 0.f), wave_sum63(mine ? v.y : 0.f),
+                       wave_sum63(mine ? v.z : 0.f), wave_sum63(mine ? v.w : 0.f));
+}
+
 __device__ __forceinline__ void f4atomic(float* p, float4 v) {
     atomicAdd(p + 0, v.x);
     atomicAdd(p + 1, v.y);
@@ -148,14 +176,10 @@ __device__ __forceinline__ void f4atomic(float* p, float4 v) {
     atomicAdd(p + 3, v.w);
 }
 
-__device__ __forceinline__ float4 f4add(float4 a, float4 b) {
-    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
-}
-
-// Corner adjoints of the accumulated moments: corner (ex, ey, ez) has weight
+// Corner adjoints of summed moments: corner (ex, ey, ez) has weight
 // prod_a (e_a ? t_a : 1 - t_a), whose monomial expansion gives these sums.
-__device__ __forceinline__ void flush(const DevGrid& G, float* __restrict__ adj, Moments& M) {
-    if (M.id < 0) return;
+__device__ __forceinline__ void scatter_corners(const DevGrid& G, float* __restrict__ adj, int id,
+                                                const Moments& M) {
     const float4 c111 = M.mxyz;
     const float4 c110 = f4sub(M.mxy, M.mxyz);
     const float4 c101 = f4sub(M.mxz, M.mxyz);
@@ -165,7 +189,7 @@ __device__ __forceinline__ void flush(const DevGrid& G, float* __restrict__ adj,
     const float4 c001 = f4sub(f4sub(M.mz, M.mxz), c011);
     const float4 c000 = f4sub(f4sub(f4sub(M.m0, M.mx), f4add(M.my, M.mz)),
                               f4sub(f4sub(f4sub(M.mxyz, M.mxy), M.mxz), M.myz));
-    float* p = adj + (int64_t)M.id * 4;
+    float* p = adj + (int64_t)id * 4;
     f4atomic(p, c000);
     f4atomic(p + 4 * G.s2, c001);
     f4atomic(p + 4 * G.s1, c010);
@@ -174,70 +198,118 @@ __device__ __forceinline__ void flush(const DevGrid& G, float* __restrict__ adj,
     f4atomic(p + 4 * (G.s0 + G.s2), c101);
     f4atomic(p + 4 * (G.s0 + G.s1), c110);
     f4atomic(p + 4 * (G.s0 + G.s1 + G.s2), c111);
-    M.id = -1;
 }
 
-// Adjoint of one RHS evaluation at p with output adjoint Ab (gx, gy, gz, kappa):
-// accumulates the grid adjoint and returns d(loss)/d(p) in rb.
-__device__ __forceinline__ void eval_adj(const DevGrid& G, float x, float y, float z,
-                                         const float4& Ab, Cell& cc, Moments& M,
-                                         float* __restrict__ adj, float rb[3]) {
-    rb[0] = rb[1] = rb[2] = 0.f;
-    if (!in_box(G, x, y, z)) return;
-    const float u0 = (x - G.lo0) * G.id0;
-    const float u1 = (y - G.lo1) * G.id1;
-    const float u2 = (z - G.lo2) * G.id2;
-    const int i0 = min((int)u0, G.n0 - 2);
-    const int i1 = min((int)u1, G.n1 - 2);
-    const int i2 = min((int)u2, G.n2 - 2);
-    const float tx = u0 - (float)i0;
-    const float ty = u1 - (float)i1;
-    const float tz = u2 - (float)i2;
-    const int id = (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2));
-    if (id != cc.id) load_cell(G, id, cc);
-    // channel-contracted coefficients: Kq = sum_ch Ab_ch * kq_ch
-    auto dot = [&](const float4& k) {
-        return fmaf(Ab.w, k.w, fmaf(Ab.z, k.z, fmaf(Ab.y, k.y, Ab.x * k.x)));
-    };
-    const float Kx = dot(cc.kx), Ky = dot(cc.ky), Kz = dot(cc.kz);
-    const float Kxy = dot(cc.kxy), Kxz = dot(cc.kxz), Kyz = dot(cc.kyz), Kxyz = dot(cc.kxyz);
-    // d f / d t_a of f = k0 + kx tx + ky ty + kz tz + kxy tx ty + kxz tx tz + kyz ty tz + kxyz tx ty tz
-    rb[0] = G.id0 * fmaf(tz, fmaf(ty, Kxyz, Kxz), fmaf(ty, Kxy, Kx));
-    rb[1] = G.id1 * fmaf(tz, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxy, Ky));
-    rb[2] = G.id2 * fmaf(ty, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxz, Kz));
-    if (id != M.id) {
-        flush(G, adj, M);
+// Collective flush (whole wavefront, uniform control flow): the lanes with
+// `need` set are grouped by cell; each group's moments are summed across the
+// wavefront and lane 63 scatters the group's corner adjoints with atomics.
+// Lanes from one sorted wavefront share a few cells, so this replaces 64
+// lanes' atomics on the same voxels with one lane's.
+__device__ __forceinline__ void flush_wave(const DevGrid& G, float* __restrict__ adj, Moments& M,
+                                           bool need) {
+    uint64_t pending = __ballot(need);
+    const int lane = __lane_id();
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int key = __builtin_amdgcn_readlane(M.id, leader);
+        const bool mine = need && M.id == key;
+        pending &= ~__ballot(mine);
+        Moments S;
+        S.m0 = wave_sum63(M.m0, mine);
+        S.mx = wave_sum63(M.mx, mine);
+        S.my = wave_sum63(M.my, mine);
+        S.mz = wave_sum63(M.mz, mine);
+        S.mxy = wave_sum63(M.mxy, mine);
+        S.mxz = wave_sum63(M.mxz, mine);
+        S.myz = wave_sum63(M.myz, mine);
+        S.mxyz = wave_sum63(M.mxyz, mine);
+        if (lane == 63) scatter_corners(G, adj, key, S);
+    }
+}
+
+// Adds one stage's adjoint Ab at cell `id`, local coordinates t, to the lane's
+// moments (contrib = this lane has a contribution).  Called by every lane of
+// the wavefront together.
+__device__ __forceinline__ void accumulate(const DevGrid& G, float* __restrict__ adj, Moments& M,
+                                           bool contrib, int id, float tx, float ty, float tz,
+                                           const float4& Ab) {
+    const bool move = contrib && id != M.id;
+    if (__ballot(move && M.id >= 0)) flush_wave(G, adj, M, move && M.id >= 0);
+    if (move) {
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = z4;
         M.id = id;
     }
-    const float txy = tx * ty, txz = tx * tz, tyz = ty * tz, txyz = txy * tz;
-    M.m0 = f4add(M.m0, Ab);
-    f4fma(M.mx, tx, Ab);
-    f4fma(M.my, ty, Ab);
-    f4fma(M.mz, tz, Ab);
-    f4fma(M.mxy, txy, Ab);
-    f4fma(M.mxz, txz, Ab);
-    f4fma(M.myz, tyz, Ab);
-    f4fma(M.mxyz, txyz, Ab);
+    if (contrib) {
+        const float txy = tx * ty, txz = tx * tz, tyz = ty * tz, txyz = txy * tz;
+        M.m0 = f4add(M.m0, Ab);
+        f4fma(M.mx, tx, Ab);
+        f4fma(M.my, ty, Ab);
+        f4fma(M.mz, tz, Ab);
+        f4fma(M.mxy, txy, Ab);
+        f4fma(M.mxz, txz, Ab);
+        f4fma(M.myz, tyz, Ab);
+        f4fma(M.mxyz, txyz, Ab);
+    }
+}
+
+// Adjoint of one RHS evaluation at p with output adjoint Ab (gx, gy, gz, kappa):
+// returns d(loss)/d(p) in rb and accumulates the grid adjoint.  Every lane of the
+// wavefront calls it together; `act` lanes contribute.
+__device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float x, float y, float z,
+                                         const float4& Ab, Cell& cc, Moments& M,
+                                         float* __restrict__ adj, float rb[3]) {
+    rb[0] = rb[1] = rb[2] = 0.f;
+    const bool in = act && in_box(G, x, y, z);
+    int id = -1;
+    float tx = 0.f, ty = 0.f, tz = 0.f;
+    if (in) {
+        const float u0 = (x - G.lo0) * G.id0;
+        const float u1 = (y - G.lo1) * G.id1;
+        const float u2 = (z - G.lo2) * G.id2;
+        const int i0 = min((int)u0, G.n0 - 2);
+        const int i1 = min((int)u1, G.n1 - 2);
+        const int i2 = min((int)u2, G.n2 - 2);
+        tx = u0 - (float)i0;
+        ty = u1 - (float)i1;
+        tz = u2 - (float)i2;
+        id = (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2));
+        if (id != cc.id) load_cell(G, id, cc);
+        // channel-contracted coefficients: Kq = sum_ch Ab_ch * kq_ch
+        auto dot = [&](const float4& k) {
+            return fmaf(Ab.w, k.w, fmaf(Ab.z, k.z, fmaf(Ab.y, k.y, Ab.x * k.x)));
+        };
+        const float Kx = dot(cc.kx), Ky = dot(cc.ky), Kz = dot(cc.kz);
+        const float Kxy = dot(cc.kxy), Kxz = dot(cc.kxz), Kyz = dot(cc.kyz), Kxyz = dot(cc.kxyz);
+        // d f / d t_a of the cell polynomial
+        rb[0] = G.id0 * fmaf(tz, fmaf(ty, Kxyz, Kxz), fmaf(ty, Kxy, Kx));
+        rb[1] = G.id1 * fmaf(tz, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxy, Ky));
+        rb[2] = G.id2 * fmaf(ty, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxz, Kz));
+    }
+    accumulate(G, adj, M, in, id, tx, ty, tz, Ab);
 }
 
 // Transposed RK4 step.  On entry rb, vb are the adjoints of the state after the
-// step; on exit, of the state before it.  tb (tau adjoint) is unchanged by a step.
+// step; on exit, of the state before it.  tb (tau adjoint) is unchanged by a
+// step.  Called by every lane together; lanes with act = false pass through.
 template <bool KAPPA>
-__device__ __forceinline__ void step_adj(const DevGrid& G, const float rh[3], const float vh[3],
-                                         float h, float rb[3], float vb[3], float tb, Cell& cc,
-                                         Moments& M, float* __restrict__ adj) {
+__device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float rh[3],
+                                         const float vh[3], float h, float rb[3], float vb[3],
+                                         float tb, Cell& cc, Moments& M,
+                                         float* __restrict__ adj) {
     const float hh = 0.5f * h;
     const float h6 = h * (1.0f / 6.0f);
     const float h3 = h * (1.0f / 3.0f);
     const float hh6 = h * h6;
     // forward stage points (the same arithmetic as integrate())
-    float4 A1 = make_float4(0.f, 0.f, 0.f, 0.f), A2;
-    if (in_box(G, rh[0], rh[1], rh[2])) eval_rhs<KAPPA, false>(G, rh[0], rh[1], rh[2], cc, A1);
+    float4 A1 = make_float4(0.f, 0.f, 0.f, 0.f), A2 = A1;
+    if (act) {
+        if (in_box(G, rh[0], rh[1], rh[2])) eval_rhs<KAPPA, false>(G, rh[0], rh[1], rh[2], cc, A1);
+        eval_rhs<KAPPA, true>(G, fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]),
+                              fmaf(hh, vh[2], rh[2]), cc, A2);
+    }
     const float r2[3] = {fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]), fmaf(hh, vh[2], rh[2])};
     const float v2[3] = {fmaf(hh, A1.x, vh[0]), fmaf(hh, A1.y, vh[1]), fmaf(hh, A1.z, vh[2])};
-    eval_rhs<KAPPA, true>(G, r2[0], r2[1], r2[2], cc, A2);
     const float r3[3] = {fmaf(hh, v2[0], rh[0]), fmaf(hh, v2[1], rh[1]), fmaf(hh, v2[2], rh[2])};
     const float v3[3] = {fmaf(hh, A2.x, vh[0]), fmaf(hh, A2.y, vh[1]), fmaf(hh, A2.z, vh[2])};
     const float r4[3] = {fmaf(h, v3[0], rh[0]), fmaf(h, v3[1], rh[1]), fmaf(h, v3[2], rh[2])};
@@ -245,93 +317,104 @@ __device__ __forceinline__ void step_adj(const DevGrid& G, const float rh[3], co
     float p4[3], p3[3], p2[3], p1[3];
     // stage 4: a4 enters v' with weight h/6
     const float4 B4 = make_float4(h6 * vb[0], h6 * vb[1], h6 * vb[2], h6 * tk);
-    eval_adj(G, r4[0], r4[1], r4[2], B4, cc, M, adj, p4);
+    eval_adj(G, act, r4[0], r4[1], r4[2], B4, cc, M, adj, p4);
     const float v3b[3] = {h * p4[0], h * p4[1], h * p4[2]};  // r4 = r + h v3
     // stage 3: a3 enters r' (h^2/6) and v' (h/3)
     const float4 B3 = make_float4(fmaf(hh6, rb[0], h3 * vb[0]), fmaf(hh6, rb[1], h3 * vb[1]),
                                   fmaf(hh6, rb[2], h3 * vb[2]), h3 * tk);
-    eval_adj(G, r3[0], r3[1], r3[2], B3, cc, M, adj, p3);
+    eval_adj(G, act, r3[0], r3[1], r3[2], B3, cc, M, adj, p3);
     const float v2b[3] = {hh * p3[0], hh * p3[1], hh * p3[2]};  // r3 = r + h/2 v2
     // stage 2: a2 enters r' (h^2/6), v' (h/3) and v3 = v + h/2 a2
     const float4 B2 = make_float4(fmaf(hh, v3b[0], fmaf(hh6, rb[0], h3 * vb[0])),
                                   fmaf(hh, v3b[1], fmaf(hh6, rb[1], h3 * vb[1])),
                                   fmaf(hh, v3b[2], fmaf(hh6, rb[2], h3 * vb[2])), h3 * tk);
-    eval_adj(G, r2[0], r2[1], r2[2], B2, cc, M, adj, p2);
+    eval_adj(G, act, r2[0], r2[1], r2[2], B2, cc, M, adj, p2);
     // stage 1: a1 enters r' (h^2/6), v' (h/6) and v2 = v + h/2 a1
     const float4 B1 = make_float4(fmaf(hh, v2b[0], fmaf(hh6, rb[0], h6 * vb[0])),
                                   fmaf(hh, v2b[1], fmaf(hh6, rb[1], h6 * vb[1])),
                                   fmaf(hh, v2b[2], fmaf(hh6, rb[2], h6 * vb[2])), h6 * tk);
-    eval_adj(G, rh[0], rh[1], rh[2], B1, cc, M, adj, p1);
+    eval_adj(G, act, rh[0], rh[1], rh[2], B1, cc, M, adj, p1);
+    if (act) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        // v: r2 = r + h/2 v, v2, v3, r' = r + h v + ..., v' = v + ...
-        const float nv = vb[c] + h * rb[c] + v3b[c] + v2b[c] + hh * p2[c];
-        rb[c] = rb[c] + p1[c] + p2[c] + p3[c] + p4[c];
-        vb[c] = nv;
+        for (int c = 0; c < 3; ++c) {
+            // v: r2 = r + h/2 v, v2, v3, r' = r + h v + ..., v' = v + ...
+            const float nv = vb[c] + h * rb[c] + v3b[c] + v2b[c] + hh * p2[c];
+            rb[c] = rb[c] + p1[c] + p2[c] + p3[c] + p4[c];
+            vb[c] = nv;
+        }
     }
 }
 
+// One ray per lane; the lanes of a wavefront walk their tapes in lockstep
+// rounds so that grid-adjoint flushes can be combined across the wavefront.
 template <bool KAPPA>
 __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
     const int lb = xcd_block(blockIdx.x, gridDim.x, A.nblocks);
-    if (lb < 0) return;
+    if (lb < 0) return;  // whole block: padding of the XCD remap
     const int64_t t = (int64_t)lb * blockDim.x + threadIdx.x;
-    if (t >= A.n) return;
+    const bool valid = t < A.n;
     const int64_t n = A.n;
-    const int64_t ray = A.order ? (int64_t)A.order[t] : t;
-    float rb[3], vb[3];
+    const int64_t tt = valid ? t : 0;  // in-bounds row for the lanes past the end
+    const int64_t ray = valid ? (A.order ? (int64_t)A.order[t] : t) : 0;
+    float rb[3] = {0.f, 0.f, 0.f}, vb[3] = {0.f, 0.f, 0.f}, v0[3] = {0.f, 0.f, 0.f};
+    float tb = 0.f;
+    if (valid) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        rb[c] = A.seed_state[c * n + ray];
-        vb[c] = A.seed_state[(3 + c) * n + ray];
+        for (int c = 0; c < 3; ++c) {
+            rb[c] = A.seed_state[c * n + ray];
+            vb[c] = A.seed_state[(3 + c) * n + ray];
+            v0[c] = A.s0[(3 + c) * n + ray];
+        }
+        tb = A.seed_tau ? (float)A.seed_tau[ray] : 0.f;
     }
-    const float tb = A.seed_tau ? (float)A.seed_tau[ray] : 0.f;
-    float v0[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) v0[c] = A.s0[(3 + c) * n + ray];
     Cell cc;
     cc.id = -1;
     Moments M;
     M.id = -1;
     const SaveSink none{nullptr, nullptr, n, ray, 0, A.nseg};
     for (int c = A.nck - 1; c >= 0; --c) {
-        const CkptView v = ckpt_view(A.ckpt, n, c);
-        RayState s;
+        GlobalTape tape{A.tape, n, tt, 0};
+        if (valid) {
+            const CkptView v = ckpt_view(A.ckpt, n, c);
+            RayState s;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            s.rb[a] = v.rb[a][t];
-            s.off[a] = v.off[a][t];
-            s.dv[a] = v.dv[a][t];
-            s.v0[a] = v0[a];
+            for (int a = 0; a < 3; ++a) {
+                s.rb[a] = v.rb[a][t];
+                s.off[a] = v.off[a][t];
+                s.dv[a] = v.dv[a][t];
+                s.v0[a] = v0[a];
+            }
+            s.tau = v.tau[t];
+            s.fold = v.fold[t];
+            s.path = 0.0;
+            s.steps = 0;
+            const int64_t g0 = (int64_t)c * A.K;
+            const int64_t g1 = g0 + A.K < A.sched ? g0 + A.K : A.sched;
+            integrate<KAPPA, false, GlobalTape>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1,
+                                                none, tape);
         }
-        s.tau = v.tau[t];
-        s.fold = v.fold[t];
-        s.path = 0.0;
-        s.steps = 0;
-        const int64_t g0 = (int64_t)c * A.K;
-        const int64_t g1 = g0 + A.K < A.sched ? g0 + A.K : A.sched;
-        GlobalTape tape{A.tape, n, t, 0};
-        integrate<KAPPA, false, GlobalTape>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1, none,
-                                            tape);
-        for (int j = tape.count - 1; j >= 0; --j) {
-            const float* p = tape.slot(j);
-            const float hT = p[6 * n + t];
-            if (hT < 0.f) {
+        for (int round = 0; __ballot(round < tape.count); ++round) {
+            const bool act = round < tape.count;
+            float rh[3] = {0.f, 0.f, 0.f}, vh[3] = {0.f, 0.f, 0.f}, hT = 0.f;
+            if (act) {
+                const float* p = tape.slot(tape.count - 1 - round);
+                hT = p[6 * n + tt];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    rh[a] = p[a * n + tt];
+                    vh[a] = p[(3 + a) * n + tt];
+                }
+            }
+            if (act && hT < 0.f) {
                 // straight move r' = r + T v
 #pragma unroll
                 for (int a = 0; a < 3; ++a) vb[a] = fmaf(-hT, rb[a], vb[a]);
-            } else {
-                float rh[3], vh[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    rh[a] = p[a * n + t];
-                    vh[a] = p[(3 + a) * n + t];
-                }
-                step_adj<KAPPA>(A.G, rh, vh, hT, rb, vb, tb, cc, M, A.grid_adj);
             }
+            step_adj<KAPPA>(A.G, act && hT > 0.f, rh, vh, hT > 0.f ? hT : 0.f, rb, vb, tb, cc,
+                            M, A.grid_adj);
         }
     }
-    flush(A.G, A.grid_adj, M);
+    flush_wave(A.G, A.grid_adj, M, M.id >= 0);
 }
 
 // ---------------------------------------------------------------------------
