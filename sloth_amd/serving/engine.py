@@ -120,8 +120,15 @@ def _f(t):
 
 
 class ServingEngine:
-    def __init__(self, *, max_slots: int = 32, hbm_budget: int | None = None,
-                 blocks: int = 1024, batch_work: float = 2e9, use_graph: bool = True,
+    """Continuous-batching engine.
+
+    ``blocks``: workgroups of the persistent step-batch kernel; 768 = one resident
+    round on MI355X (256 CUs x 3 workgroups of 4 waves at its register count).
+    ``batch_work``: target ray-steps per batch (the batch length in steps is this
+    over the rays resident); smaller batches admit new requests sooner.
+    """
+    def __init__(self, *, max_slots: int = 16, hbm_budget: int | None = None,
+                 blocks: int = 768, batch_work: float = 5e8, use_graph: bool = True,
                  step_fraction: float = 0.5, keep_outputs: bool = True, device=None):
         self.dev = device if device is not None else N.require_device()
         if hbm_budget is None:
