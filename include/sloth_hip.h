@@ -121,7 +121,7 @@ typedef struct slt_trace_args {
     const float* seg_h;     /* device [n_seg], seconds                                     */
     float* save_state;      /* device (n_seg+1) x 6 x n_rays (or 1 x 6 x n_rays if save_every == 0) */
     double* save_tau;       /* device, same leading shape as save_state, 1 row; or NULL    */
-    double* path;           /* device [n_rays] optical path length [m] or NULL             */
+    double* path;           /* device [n_rays] path length [m] (polyline through the step end points) or NULL */
     uint32_t* steps;        /* device [n_rays] in-box RK4 steps per ray or NULL            */
     unsigned long long* total_steps; /* device [1], incremented by all in-box steps, or NULL */
 } slt_trace_args;
@@ -131,8 +131,9 @@ typedef struct slt_trace_args {
  * diffrax solve of src/core/propagator.py:726-793 and :1162-1198 with the RHS
  * dsdt (:168-243) and trilinear interpolation (src/core/interpolator.py:15-106):
  *   dr/dt = v,  dv/dt = interp(grad(coeff ne))(r),  dtau/dt = interp(kappa)(r)
- * with the interpolant 0 outside the closed grid box.  Positions and velocities
- * are carried as compensated fp32 pairs; tau and path accumulate in fp64.
+ * with the interpolant 0 outside the closed grid box.  Position is an fp64 base
+ * plus an fp32 offset, velocity the start velocity plus an fp32 change; tau and
+ * path length accumulate in fp64.
  */
 int slt_trace(const void* packed, const slt_grid_desc* g, const slt_trace_args* a,
               slt_stream_t stream);

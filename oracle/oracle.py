@@ -153,7 +153,8 @@ def interp_points(coords, values, pts):
 def trace(grads, kap, coords, s0, nsub, h, naive=False, nthreads=0):
     """Fixed-step RK4 trace.  Returns (saves[(S+1),8,N], steps[N]).
 
-    Rows of each save: x, y, z, vx, vy, vz, tau, path.
+    Rows of each save: x, y, z, vx, vy, vz, tau, path (length of the polyline through
+    the step end points).
     """
     lib = load()
     dndx, dndy, dndz = (_f64(g) for g in grads)
@@ -260,11 +261,11 @@ def np_trace(grads, kap, coords, s0, nsub, h):
             r4 = r + hh * v3
             v4 = v + hh * a3
             a4, k4, i4 = rhs(r4)
-            sp = [np.linalg.norm(q, axis=1) for q in (v, v2, v3, v4)]
-            r = r + hh / 6.0 * (v + 2.0 * v2 + 2.0 * v3 + v4)
+            dr = hh / 6.0 * (v + 2.0 * v2 + 2.0 * v3 + v4)
+            r = r + dr
             v = v + hh / 6.0 * (a1 + 2.0 * a2 + 2.0 * a3 + a4)
             tau = tau + hh / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
-            path = path + hh / 6.0 * (sp[0] + 2.0 * sp[1] + 2.0 * sp[2] + sp[3])
+            path = path + np.linalg.norm(dr, axis=1)
             steps += (i1 | i2 | i3 | i4)
         saves.append(np.vstack([r.T, v.T, tau, path]))
     return np.stack(saves), steps

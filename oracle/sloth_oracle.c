@@ -253,16 +253,15 @@ void orc_trace(int nx, int ny, int nz, const double* cx, const double* cy, const
                 orc_rhs(&G, r3, a3, &k3, &inside);
                 for (int c = 0; c < 3; ++c) { r4[c] = r[c] + hh * v3[c]; v4[c] = v[c] + hh * a3[c]; }
                 orc_rhs(&G, r4, a4, &k4, &inside);
-                const double sp1 = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                const double sp2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
-                const double sp3 = sqrt(v3[0] * v3[0] + v3[1] * v3[1] + v3[2] * v3[2]);
-                const double sp4 = sqrt(v4[0] * v4[0] + v4[1] * v4[1] + v4[2] * v4[2]);
+                double dr2 = 0.0;
                 for (int c = 0; c < 3; ++c) {
-                    r[c] += hh / 6.0 * (v[c] + 2.0 * v2[c] + 2.0 * v3[c] + v4[c]);
+                    const double dr = hh / 6.0 * (v[c] + 2.0 * v2[c] + 2.0 * v3[c] + v4[c]);
+                    r[c] += dr;
+                    dr2 += dr * dr;
                     v[c] += hh / 6.0 * (a1[c] + 2.0 * a2[c] + 2.0 * a3[c] + a4[c]);
                 }
                 tau += hh / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
-                path += hh / 6.0 * (sp1 + 2.0 * sp2 + 2.0 * sp3 + sp4);
+                path += sqrt(dr2);
                 nstep += inside;
             }
             ORC_SAVE(s + 1);

@@ -236,23 +236,18 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                             v3z = fmaf(hh, A2.z, vh[2]);
                 in |= eval_rhs<KAPPA, true>(G, fmaf(hh, v2x, rh[0]), fmaf(hh, v2y, rh[1]),
                                             fmaf(hh, v2z, rh[2]), cc, A3);
-                const float v4x = fmaf(h, A3.x, vh[0]), v4y = fmaf(h, A3.y, vh[1]),
-                            v4z = fmaf(h, A3.z, vh[2]);
                 in |= eval_rhs<KAPPA, true>(G, fmaf(h, v3x, rh[0]), fmaf(h, v3y, rh[1]),
                                             fmaf(h, v3z, rh[2]), cc, A4);
-                if (PATH) {
-                    const float s1 = __builtin_amdgcn_sqrtf(
-                        fmaf(vh[0], vh[0], fmaf(vh[1], vh[1], vh[2] * vh[2])));
-                    const float s2 = __builtin_amdgcn_sqrtf(fmaf(v2x, v2x, fmaf(v2y, v2y, v2z * v2z)));
-                    const float s3 = __builtin_amdgcn_sqrtf(fmaf(v3x, v3x, fmaf(v3y, v3y, v3z * v3z)));
-                    const float s4 = __builtin_amdgcn_sqrtf(fmaf(v4x, v4x, fmaf(v4y, v4y, v4z * v4z)));
-                    s.path += (double)(h6 * (s1 + 2.f * (s2 + s3) + s4));
-                }
                 // r += h v + h^2/6 (A1 + A2 + A3)   (= h/6 (v1 + 2 v2 + 2 v3 + v4))
                 // v += h/6 (A1 + 2 A2 + 2 A3 + A4)
-                s.off[0] += fmaf(h, vh[0], hh6 * (A1.x + A2.x + A3.x));
-                s.off[1] += fmaf(h, vh[1], hh6 * (A1.y + A2.y + A3.y));
-                s.off[2] += fmaf(h, vh[2], hh6 * (A1.z + A2.z + A3.z));
+                const float dx = fmaf(h, vh[0], hh6 * (A1.x + A2.x + A3.x));
+                const float dy = fmaf(h, vh[1], hh6 * (A1.y + A2.y + A3.y));
+                const float dz = fmaf(h, vh[2], hh6 * (A1.z + A2.z + A3.z));
+                s.off[0] += dx;
+                s.off[1] += dy;
+                s.off[2] += dz;
+                // path length: the polyline through the step end points
+                if (PATH) s.path += (double)__builtin_amdgcn_sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
                 s.dv[0] += h6 * (A1.x + 2.f * (A2.x + A3.x) + A4.x);
                 s.dv[1] += h6 * (A1.y + 2.f * (A2.y + A3.y) + A4.y);
                 s.dv[2] += h6 * (A1.z + 2.f * (A2.z + A3.z) + A4.z);
