@@ -213,30 +213,44 @@ typedef struct slt_job {
     unsigned long long* step_counter; /* += in-box steps taken by this batch, or NULL */
 } slt_job;
 
+/* Unit queues of a step batch: one per XCD (a workgroup's XCD is its index mod 8). */
+#define SLT_WORK_QUEUES 8
+
 typedef struct slt_job_table {
+    /* Work queues of slt_advance, one 128-byte line each so the XCDs do not share a
+     * line: next_unit[q][0] is the next unit queue q hands out, waves_done[0] the
+     * wavefronts that have drained every queue.  Zero when a launch starts
+     * (slt_job_table_prepare zeroes them; the launch's last wavefront resets them). */
+    uint32_t next_unit[SLT_WORK_QUEUES][32];
+    uint32_t waves_done[32];
     int32_t n_jobs;
     int32_t n_units;                      /* filled by slt_job_table_prepare */
     int32_t unit_begin[SLT_MAX_JOBS + 1]; /* filled by slt_job_table_prepare */
+    int32_t reserved;
     slt_job jobs[SLT_MAX_JOBS];
 } slt_job_table;
 
 /*
- * Host-only: validate every job (grid, pointers, step ranges) and fill
- * n_units / unit_begin (one unit = 256 rays).  Call before copying the table
- * to the device.
+ * Host-only: validate every job (grid, pointers, step ranges), fill n_units /
+ * unit_begin (one unit = 64 rays, one wavefront) and zero the work queues.
+ * Call before copying the table to the device.
  */
 int slt_job_table_prepare(slt_job_table* host_table);
 
 /*
  * One step batch: advance every job of a device-resident, prepared table by
  * its [step_begin, step_end) range, in one persistent launch of `blocks`
- * workgroups (a positive multiple of 8).  The kernel arguments do not depend
- * on the table's contents, so a launch can be captured in a HIP graph once and
+ * workgroups (a positive multiple of 8).  Each wavefront takes units from its
+ * XCD's queue (queue q holds the q-th eighth of the units, in ray order) and,
+ * when that is empty, from the other queues, so the batch ends when the work
+ * does, whatever the mix of jobs.  The kernel arguments do not depend on the
+ * table's contents, so a launch can be captured in a HIP graph once and
  * replayed after each table update.  A request advanced in several batches
  * gives the same result as one slt_trace call (up to fp64 rounding of the
  * straight-line moves outside the grid, which are split at batch ends).
+ * Launches on one table must not overlap.
  */
-int slt_advance(const slt_job_table* device_table, int32_t blocks, slt_stream_t stream);
+int slt_advance(slt_job_table* device_table, int32_t blocks, slt_stream_t stream);
 
 /* ------------------------------------------------------------------------- */
 /* Adjoint: d(loss on final ray states)/d(n_e).  No reference counterpart     */

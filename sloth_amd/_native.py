@@ -26,6 +26,7 @@ EXPORTED = (
 )
 
 SLT_MAX_JOBS = 64
+SLT_WORK_QUEUES = 8
 
 
 class GridDesc(ctypes.Structure):
@@ -102,9 +103,12 @@ class Job(ctypes.Structure):
 
 class JobTable(ctypes.Structure):
     _fields_ = [
+        ("next_unit", (ctypes.c_uint32 * 32) * SLT_WORK_QUEUES),
+        ("waves_done", ctypes.c_uint32 * 32),
         ("n_jobs", ctypes.c_int32),
         ("n_units", ctypes.c_int32),
         ("unit_begin", ctypes.c_int32 * (SLT_MAX_JOBS + 1)),
+        ("reserved", ctypes.c_int32),
         ("jobs", Job * SLT_MAX_JOBS),
     ]
 

@@ -42,10 +42,29 @@ def host_f32(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
 
 
-def device_f32(a, device) -> torch.Tensor:
+_STAGED_MAX_BYTES = 1 << 20
+
+
+def to_device(a, device, stream=None) -> torch.Tensor:
+    """Host array -> device tensor, queued on ``stream`` (default: the current one).
+
+    Arrays up to 1 MiB (coordinates, schedules, bin edges) go through pinned
+    memory without blocking the host; torch's caching host allocator keeps the
+    staging block until the copy has run.  A plain ``.to(device)`` would wait
+    for everything queued on the stream before it, which stalls a host that is
+    feeding several streams.  Larger arrays take the ordinary (blocking) copy.
+    """
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    with torch.cuda.stream(stream):
+        if t.numel() * t.element_size() <= _STAGED_MAX_BYTES:
+            return t.pin_memory().to(device, non_blocking=True)
+        return t.to(device)
+
+
+def device_f32(a, device, stream=None) -> torch.Tensor:
     if isinstance(a, torch.Tensor):
         return a.detach().to(device=device, dtype=torch.float32).contiguous()
-    return torch.from_numpy(host_f32(a)).to(device)
+    return to_device(host_f32(a), device, stream)
 
 
 def _cl(coulomb_log) -> float:
@@ -70,7 +89,7 @@ class PackedGrid:
         self.inv_brems = bool(inv_brems)
         self.coords = [host_f32(c).reshape(-1) for c in (x, y, z)]
         nx, ny, nz = (c.size for c in self.coords)
-        ne_d = device_f32(ne, dev)
+        ne_d = device_f32(ne, dev, stream)
         if tuple(ne_d.shape) != (nx, ny, nz):
             raise AssertionError(
                 f"ne has shape {tuple(ne_d.shape)} but the coordinates give {(nx, ny, nz)}")
@@ -82,13 +101,13 @@ class PackedGrid:
         self.shape = (nx, ny, nz)
         self.nbytes = int(L.slt_grid_bytes(ctypes.byref(desc)))
         self.voxels = torch.empty(self.nbytes // 4, dtype=torch.float32, device=dev)
-        cd = [torch.from_numpy(c).to(dev) for c in self.coords]
+        cd = [to_device(c, dev, stream) for c in self.coords]
         te_d = z_d = None
         if self.inv_brems:
             if Te is None or Z is None:
                 raise AssertionError("Te and Z must both be provided when inv_brems=True.")
-            te_d = device_f32(Te, dev).reshape(-1)
-            z_d = device_f32(Z, dev).reshape(-1)
+            te_d = device_f32(Te, dev, stream).reshape(-1)
+            z_d = device_f32(Z, dev, stream).reshape(-1)
         s = N.stream_handle(stream)
         N.check(L.slt_grid_pack(
             ne_d.data_ptr(), cd[0].data_ptr(), cd[1].data_ptr(), cd[2].data_ptr(),
@@ -126,7 +145,7 @@ def grid_fields(ne, x, y, z, omega, *, Te=None, Z=None, coulomb_log=None, want_k
                                  ctypes.byref(desc)))
     outs = [torch.empty((nx, ny, nz), dtype=torch.float32, device=dev)
             for _ in range(4 if want_kappa else 3)]
-    cd = [torch.from_numpy(c).to(dev) for c in coords]
+    cd = [to_device(c, dev) for c in coords]
     te_d = z_d = None
     if want_kappa:
         te_d = device_f32(Te, dev).reshape(-1)
@@ -240,8 +259,8 @@ def trace(grid: PackedGrid, s0, save_times, *, step_fraction: float = 0.5,
         return TraceResult(states, tau, path, steps, total, None, nsub, h)
     if order is None and sort:
         order = ray_order(grid, rays, stream)
-    seg_nsub = torch.from_numpy(nsub).to(dev)
-    seg_h = torch.from_numpy(h.astype(np.float32)).to(dev)
+    seg_nsub = to_device(nsub, dev, stream)
+    seg_h = to_device(h.astype(np.float32), dev, stream)
     a = N.TraceArgs()
     a.n_rays = n
     a.s0 = rays.data_ptr()

@@ -19,16 +19,24 @@ __device__ __forceinline__ float4 f4sub(float4 a, float4 b) {
     return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
 }
 
+typedef float slt_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 voxel(const SLT_GLOBAL slt_f4* __restrict__ p, int i) {
+    const slt_f4 v = p[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ void load_cell(const DevGrid& G, int id, Cell& c) {
-    const float4* __restrict__ p = G.vox + id;
-    const float4 c000 = p[0];
-    const float4 c001 = p[G.s2];
-    const float4 c010 = p[G.s1];
-    const float4 c011 = p[G.s1 + G.s2];
-    const float4 c100 = p[G.s0];
-    const float4 c101 = p[G.s0 + G.s2];
-    const float4 c110 = p[G.s0 + G.s1];
-    const float4 c111 = p[G.s0 + G.s1 + G.s2];
+    const SLT_GLOBAL slt_f4* __restrict__ p =
+        to_global(reinterpret_cast<const slt_f4*>(G.vox)) + id;
+    const float4 c000 = voxel(p, 0);
+    const float4 c001 = voxel(p, G.s2);
+    const float4 c010 = voxel(p, G.s1);
+    const float4 c011 = voxel(p, G.s1 + G.s2);
+    const float4 c100 = voxel(p, G.s0);
+    const float4 c101 = voxel(p, G.s0 + G.s2);
+    const float4 c110 = voxel(p, G.s0 + G.s1);
+    const float4 c111 = voxel(p, G.s0 + G.s1 + G.s2);
     const float4 dz00 = f4sub(c001, c000), dz01 = f4sub(c011, c010);
     const float4 dz10 = f4sub(c101, c100), dz11 = f4sub(c111, c110);
     c.k0 = c000;

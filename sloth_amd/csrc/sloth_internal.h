@@ -68,6 +68,20 @@ __host__ __device__ inline void make_dev_grid_device(const void* packed, const s
     d->nchan = g->nchan;
 }
 
+// Device memory seen as the global address space.  A pointer that a kernel
+// reads out of memory (a job table) is generic to the compiler, which then
+// emits flat loads; through these the accesses become global loads, as for
+// pointers passed as kernel arguments.
+#define SLT_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const SLT_GLOBAL T* to_global(const T* p) {
+    return (const SLT_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ SLT_GLOBAL T* to_global(T* p) {
+    return (SLT_GLOBAL T*)p;
+}
+
 // Number of XCDs on MI355X; workgroups are dispatched to XCDs round-robin by
 // workgroup id, so block b runs on XCD (b % kXcds).
 constexpr int kXcds = 8;

@@ -28,6 +28,8 @@
 //    sorted rays (its own L2 holds that range's grid columns).
 #include <math.h>
 #include <stdio.h>
+#include <stddef.h>
+#include <string.h>
 
 #include "integrate.h"
 
@@ -78,31 +80,31 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceK K) {
 //   rb[3] f64 | tau f64 | path f64 | off[3] f32 | v0[3] f32 | dv[3] f32 | steps u32 | fold u32
 // ---------------------------------------------------------------------------
 struct StateView {
-    double* rb[3];
-    double* tau;
-    double* path;
-    float* off[3];
-    float* v0[3];
-    float* dv[3];
-    uint32_t* steps;
-    uint32_t* fold;
+    SLT_GLOBAL double* rb[3];
+    SLT_GLOBAL double* tau;
+    SLT_GLOBAL double* path;
+    SLT_GLOBAL float* off[3];
+    SLT_GLOBAL float* v0[3];
+    SLT_GLOBAL float* dv[3];
+    SLT_GLOBAL uint32_t* steps;
+    SLT_GLOBAL uint32_t* fold;
 };
 
 __host__ __device__ inline int64_t state_bytes(int64_t n) { return n * (5 * 8 + 9 * 4 + 2 * 4); }
 
 __device__ __forceinline__ StateView state_view(void* base, int64_t n) {
     StateView v;
-    double* d = static_cast<double*>(base);
+    SLT_GLOBAL double* d = to_global(static_cast<double*>(base));
     for (int c = 0; c < 3; ++c) v.rb[c] = d + c * n;
     v.tau = d + 3 * n;
     v.path = d + 4 * n;
-    float* f = reinterpret_cast<float*>(d + 5 * n);
+    SLT_GLOBAL float* f = reinterpret_cast<SLT_GLOBAL float*>(d + 5 * n);
     for (int c = 0; c < 3; ++c) {
         v.off[c] = f + c * n;
         v.v0[c] = f + (3 + c) * n;
         v.dv[c] = f + (6 + c) * n;
     }
-    uint32_t* u = reinterpret_cast<uint32_t*>(f + 9 * n);
+    SLT_GLOBAL uint32_t* u = reinterpret_cast<SLT_GLOBAL uint32_t*>(f + 9 * n);
     v.steps = u;
     v.fold = u + n;
     return v;
@@ -149,25 +151,31 @@ __global__ void __launch_bounds__(256) state_init_kernel(void* base, int64_t n,
     for (int c = 0; c < 3; ++c) v.v0[c][t] = s.v0[c];
 }
 
-// Work unit = 256 consecutive (sorted) rays of one job.  Units are split into
-// kXcds contiguous chunks; block b (on XCD b % kXcds) walks chunk b % kXcds.
+// Work unit = 64 consecutive (sorted) rays of one job: one wavefront.  The
+// units are dealt into SLT_WORK_QUEUES contiguous queues; a wavefront takes
+// units one at a time from its XCD's queue (workgroup index mod 8, the order
+// in which workgroups are dispatched to XCDs) and, once that queue is empty,
+// from the others.  A batch thus ends when its work does, however unevenly the
+// jobs' per-step costs (absorption or not, grid size) fall across the queues.
 template <bool KAPPA>
-__device__ void advance_unit(const slt_job& J, int64_t first) {
-    const int64_t t = first + threadIdx.x;
+__device__ void advance_unit(const slt_job& J, int64_t t) {
     uint32_t nsteps = 0;
     if (t < J.n_rays) {
         DevGrid G;
         make_dev_grid_device(J.packed, &J.grid, &G);
-        const StateView v = state_view(J.state, J.n_rays);
         RayState s;
-        state_load(v, t, s);
+        state_load(state_view(J.state, J.n_rays), t, s);
         const uint32_t before = s.steps;
         const bool finishing = J.step_end >= J.sched_steps;
         const int64_t ray = J.order ? (int64_t)J.order[t] : t;
         SaveSink sink{finishing ? J.out_state : nullptr, J.out_tau, J.n_rays, ray, 0, J.n_seg};
         integrate<KAPPA, true>(G, s, J.seg_nsub, J.seg_h, J.n_seg, J.step_begin, J.step_end,
                                sink);
-        state_store(v, t, s);
+        // Recompute the row's 14 addresses for the store instead of keeping the
+        // load's live through the integration (the empty asm hides that they match).
+        int64_t row = t;
+        asm volatile("" : "+v"(row));
+        state_store(state_view(J.state, J.n_rays), row, s);
         if (finishing) {
             if (J.out_path) J.out_path[ray] = s.path;
             if (J.out_steps) J.out_steps[ray] = s.steps;
@@ -177,24 +185,62 @@ __device__ void advance_unit(const slt_job& J, int64_t first) {
     if (J.step_counter) add_steps(J.step_counter, nsteps);
 }
 
-__global__ void __launch_bounds__(256) advance_kernel(const slt_job_table* __restrict__ T) {
-    const int n_units = T->n_units;
-    const int n_jobs = T->n_jobs;
-    const int chunk = kXcds > 1 ? (int)(blockIdx.x % kXcds) : 0;
-    const int per_chunk = (n_units + kXcds - 1) / kXcds;
-    const int u_lo = chunk * per_chunk;
-    const int u_hi = min(n_units, u_lo + per_chunk);
-    const int stride = (int)(gridDim.x / kXcds);
-    for (int u = u_lo + (int)(blockIdx.x / kXcds); u < u_hi; u += stride) {
-        int j = 0;
-        while (j + 1 < n_jobs && T->unit_begin[j + 1] <= u) ++j;
-        const slt_job& J = T->jobs[j];
-        if (J.step_end <= J.step_begin) continue;
-        const int64_t first = (int64_t)(u - T->unit_begin[j]) * 256;
-        if (J.grid.nchan >= 4)
-            advance_unit<true>(J, first);
+// Index of the job that owns unit u: the last j with unit_begin[j] <= u.
+__device__ __forceinline__ int job_of_unit(const slt_job_table* __restrict__ T, int u) {
+    int lo = 0, hi = T->n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (T->unit_begin[mid] <= u)
+            lo = mid;
         else
-            advance_unit<false>(J, first);
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// The table is read through T and its work queues are updated through
+// `queues` (the same allocation, disjoint bytes): with the two kept apart the
+// compiler knows the job entries do not change during the launch and reads
+// them with scalar loads into SGPRs.  Four wavefronts per SIMD, as the offline
+// kernel runs: the few values that do not fit in 128 VGPRs spill only around a
+// unit, not inside the step loop.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+advance_kernel(const slt_job_table* __restrict__ T, uint32_t* __restrict__ queues) {
+    const int n_units = T->n_units;
+    const int per_queue = (n_units + SLT_WORK_QUEUES - 1) / SLT_WORK_QUEUES;
+    const int home = (int)(blockIdx.x % SLT_WORK_QUEUES);
+    const int lane = (int)(threadIdx.x & 63);
+    for (int k = 0; k < SLT_WORK_QUEUES; ++k) {
+        const int q = (home + k) % SLT_WORK_QUEUES;
+        const int lo = q * per_queue;
+        const int len = min(n_units - lo, per_queue);
+        if (len <= 0) continue;
+        for (;;) {
+            // every lane of the wavefront is active here
+            uint32_t i = 0;
+            if (lane == 0) i = atomicAdd(&queues[q * 32], 1u);
+            i = (uint32_t)__builtin_amdgcn_readfirstlane((int)i);
+            if (i >= (uint32_t)len) break;
+            const int u = lo + (int)i;
+            const int j = job_of_unit(T, u);
+            const slt_job& J = T->jobs[j];
+            if (J.step_end <= J.step_begin) continue;
+            const int64_t t = (int64_t)(u - T->unit_begin[j]) * 64 + lane;
+            if (J.grid.nchan >= 4)
+                advance_unit<true>(J, t);
+            else
+                advance_unit<false>(J, t);
+        }
+    }
+    // The last wavefront to drain the queues resets them, so the table can be
+    // launched again without a new upload.
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t waves = gridDim.x * (blockDim.x / 64);
+        if (atomicAdd(&queues[SLT_WORK_QUEUES * 32], 1u) == waves - 1) {
+            for (int q = 0; q < SLT_WORK_QUEUES; ++q) atomicExch(&queues[q * 32], 0u);
+            atomicExch(&queues[SLT_WORK_QUEUES * 32], 0u);
+        }
     }
 }
 
@@ -302,7 +348,7 @@ extern "C" int slt_job_table_prepare(slt_job_table* t) {
             return SLT_ERR_INVALID;
         }
         t->unit_begin[j] = (int32_t)units;
-        units += (J.n_rays + 255) / 256;
+        units += (J.n_rays + 63) / 64;
         if (units > INT32_MAX / 2) {
             set_error("slt_job_table_prepare: too many rays in one table");
             return SLT_ERR_UNSUPPORTED;
@@ -310,17 +356,22 @@ extern "C" int slt_job_table_prepare(slt_job_table* t) {
     }
     t->unit_begin[t->n_jobs] = (int32_t)units;
     t->n_units = (int32_t)units;
+    memset(t->next_unit, 0, sizeof(t->next_unit));
+    memset(t->waves_done, 0, sizeof(t->waves_done));
     return SLT_OK;
 }
 
-extern "C" int slt_advance(const slt_job_table* device_table, int32_t blocks,
+extern "C" int slt_advance(slt_job_table* device_table, int32_t blocks,
                            slt_stream_t stream) {
     clear_error();
     if (!device_table || blocks < kXcds || blocks % kXcds) {
         set_error("slt_advance: need a device table and a positive multiple of %d blocks", kXcds);
         return SLT_ERR_INVALID;
     }
+    static_assert(offsetof(slt_job_table, next_unit) == 0 &&
+                      offsetof(slt_job_table, waves_done) == SLT_WORK_QUEUES * 32 * 4,
+                  "advance_kernel addresses the work queues as uint32 lines from the table base");
     hipLaunchKernelGGL(advance_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
-                       device_table);
+                       device_table, &device_table->next_unit[0][0]);
     return check_launch("slt_advance");
 }

@@ -122,13 +122,13 @@ def _f(t):
 class ServingEngine:
     """Continuous-batching engine.
 
-    ``blocks``: workgroups of the persistent step-batch kernel; 768 = one resident
-    round on MI355X (256 CUs x 3 workgroups of 4 waves at its register count).
+    ``blocks``: workgroups of the persistent step-batch kernel; 1024 = one resident
+    round on MI355X (256 CUs x 4 SIMDs x 4 wavefronts at its register count).
     ``batch_work``: target ray-steps per batch (the batch length in steps is this
     over the rays resident); smaller batches admit new requests sooner.
     """
     def __init__(self, *, max_slots: int = 16, hbm_budget: int | None = None,
-                 blocks: int = 768, batch_work: float = 5e8, use_graph: bool = True,
+                 blocks: int = 1024, batch_work: float = 1e9, use_graph: bool = True,
                  step_fraction: float = 0.5, keep_outputs: bool = True, device=None):
         self.dev = device if device is not None else N.require_device()
         if hbm_budget is None:
@@ -190,8 +190,8 @@ class ServingEngine:
                                 device=self.dev)
             N.check(N.lib().slt_ray_state_init(state.data_ptr(), n, mat.rays.data_ptr(),
                                                order.data_ptr(), N.stream_handle()))
-            seg_nsub = torch.from_numpy(nsub).to(self.dev)
-            seg_h = torch.from_numpy(h.astype(np.float32)).to(self.dev)
+            seg_nsub = engine.to_device(nsub, self.dev)
+            seg_h = engine.to_device(h.astype(np.float32), self.dev)
             out_state = torch.empty((6, n), dtype=torch.float32, device=self.dev)
             out_tau = torch.zeros(n, dtype=torch.float64, device=self.dev)
             out_path = torch.empty(n, dtype=torch.float64, device=self.dev)
@@ -242,6 +242,35 @@ class ServingEngine:
         a.done = torch.cuda.Event(enable_timing=True)
         a.done.record(self.main)
 
+    def _launch_batch(self, runnable, inflight):
+        """Queue one step batch over the set-up requests on the compute stream."""
+        n_rays = sum(a.mat.rays.shape[1] for a in runnable)
+        k = int(max(16, min(4096, self.batch_work / max(n_rays, 1))))
+        table = N.JobTable()
+        table.n_jobs = len(runnable)
+        spans = []
+        for i, a in enumerate(runnable):
+            end = a.sched if a.cursor >= a.inbox_estimate else min(a.sched, a.cursor + k)
+            table.jobs[i] = self._job(a, a.cursor, end)
+            spans.append(end)
+        N.check(N.lib().slt_job_table_prepare(ctypes.byref(table)))
+        with torch.cuda.stream(self.main):
+            self._upload(table)
+            if self.graph is not None:
+                self.graph.replay()
+            else:
+                N.check(N.lib().slt_advance(self.table_dev.data_ptr(), self.blocks,
+                                            N.stream_handle()))
+            self.batches += 1
+            ev = torch.cuda.Event()
+            ev.record(self.main)
+            inflight.append(ev)
+            for a, end in zip(runnable, spans):
+                a.cursor = end
+                if end >= a.sched:
+                    a.finished = True
+                    self._finalize(a)
+
     # -- main loop ------------------------------------------------------------
     def run(self, requests, *, realtime: bool = True, max_lookahead: int = 2):
         """Serve ``requests`` (arrival times in seconds from the start).
@@ -249,7 +278,6 @@ class ServingEngine:
         Returns (responses by rid, stats dict).  With ``realtime=False`` every
         request is treated as already arrived.
         """
-        L = N.lib()
         queue = deque(sorted(requests, key=lambda r: (r.arrival if realtime else 0.0, r.rid)))
         if not realtime:
             for r in queue:
@@ -257,55 +285,36 @@ class ServingEngine:
         adm = Admission(self.budget, self.max_slots)
         responses: dict[int, Response] = {}
         active: list[_Active] = []
+        pending = deque()   # admitted, not yet set up
         inflight = deque()
         torch.cuda.synchronize(self.dev)
         t_start = time.perf_counter()
         start_ev = torch.cuda.Event(enable_timing=True)
         start_ev.record(self.main)
         total_steps = 0
-        while queue or active:
+        while queue or active or pending:
             now = time.perf_counter() - t_start
             admitted, rejected = adm.admit(queue, now)
             for r in rejected:
                 responses[r.rid] = Response(r.rid, "rejected",
                                             f"footprint {r.footprint_bytes()} B exceeds the HBM "
                                             f"budget {self.budget} B", arrival=r.arrival)
-            for r in admitted:
+            pending.extend(admitted)
+            progressed = False
+            runnable = [a for a in active if not a.finished and a.ready.query()]
+            runnable = runnable[:N.SLT_MAX_JOBS]
+            if runnable:
+                self._launch_batch(runnable, inflight)
+                progressed = True
+            # One setup per pass, after the batch is queued: the GPU integrates
+            # while the host builds the next request's launches.
+            if pending:
+                r = pending.popleft()
                 try:
                     active.append(self._setup(r))
                 except (ValueError, NotImplementedError, N.SlothError) as e:
                     adm.release(r)
                     responses[r.rid] = Response(r.rid, "error", str(e), arrival=r.arrival)
-            runnable = [a for a in active if not a.finished and a.ready.query()]
-            runnable = runnable[:N.SLT_MAX_JOBS]
-            progressed = False
-            if runnable:
-                n_rays = sum(a.mat.rays.shape[1] for a in runnable)
-                k = int(max(16, min(4096, self.batch_work / max(n_rays, 1))))
-                table = N.JobTable()
-                table.n_jobs = len(runnable)
-                spans = []
-                for i, a in enumerate(runnable):
-                    end = a.sched if a.cursor >= a.inbox_estimate else min(a.sched, a.cursor + k)
-                    table.jobs[i] = self._job(a, a.cursor, end)
-                    spans.append(end)
-                N.check(L.slt_job_table_prepare(ctypes.byref(table)))
-                with torch.cuda.stream(self.main):
-                    self._upload(table)
-                    if self.graph is not None:
-                        self.graph.replay()
-                    else:
-                        N.check(L.slt_advance(self.table_dev.data_ptr(), self.blocks,
-                                              N.stream_handle()))
-                    self.batches += 1
-                    ev = torch.cuda.Event()
-                    ev.record(self.main)
-                    inflight.append(ev)
-                    for a, end in zip(runnable, spans):
-                        a.cursor = end
-                        if end >= a.sched:
-                            a.finished = True
-                            self._finalize(a)
                 progressed = True
             for a in [a for a in active if a.finished and a.done is not None and a.done.query()]:
                 steps = int(a.counter.item())

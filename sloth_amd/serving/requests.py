@@ -72,7 +72,7 @@ def field_on_device(spec: FieldSpec, device) -> torch.Tensor:
     """(n, n, n) float32 electron density [m^-3] built on the device."""
     n = spec.side
     if spec.kind == "sine":
-        x = torch.from_numpy(_coords(spec)[0].astype(np.float64)).to(device)
+        x = engine.to_device(_coords(spec)[0].astype(np.float64), device)
         prof = spec.n0 * (1.0 + spec.frac * torch.sin(2 * math.pi * x / spec.wavelength))
         return prof.to(torch.float32)[:, None, None].expand(n, n, n).contiguous()
     if spec.kind != "kolmogorov":

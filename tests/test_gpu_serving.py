@@ -101,3 +101,68 @@ def test_poisson_arrivals_latency_reported():
     assert stats["ok"] == 6
     assert all(resp[r.rid].latency > 0 for r in reqs)
     assert stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
+
+
+def test_advance_queues_drain_and_reset():
+    """Eight workgroups (one per queue) take every unit; the last wavefront
+    leaves the queues at zero; the result equals the offline trace."""
+    import ctypes
+
+    import torch
+
+    from sloth_amd import _native as N
+    from sloth_amd.core import engine
+    from sloth_amd.serving import BeamSpec, FieldSpec, RequestSpec
+    from sloth_amd.serving.requests import materialize
+
+    dev = torch.device("cuda:0")
+    req = RequestSpec(rid=0, field=FieldSpec(kind="kolmogorov", side=40, length=6e-3, seed=5,
+                                             n0=4e24, frac=0.3, l_min=0.4e-3),
+                      beam=BeamSpec(n_rays=5000, half_width=2e-3, seed=6), bins=32)
+    mat = materialize(req, dev)
+    n = mat.rays.shape[1]
+    L = N.lib()
+    nsub, h = engine.schedule(mat.save_times, engine.nominal_step(mat.grid, 0.5))
+    order = engine.ray_order(mat.grid, mat.rays)
+    state = torch.empty(int(L.slt_ray_state_bytes(n)), dtype=torch.uint8, device=dev)
+    N.check(L.slt_ray_state_init(state.data_ptr(), n, mat.rays.data_ptr(), order.data_ptr(),
+                                 N.stream_handle()))
+    seg_nsub = torch.from_numpy(nsub).to(dev)
+    seg_h = torch.from_numpy(h.astype(np.float32)).to(dev)
+    out_state = torch.empty((6, n), dtype=torch.float32, device=dev)
+    out_tau = torch.zeros(n, dtype=torch.float64, device=dev)
+    out_path = torch.empty(n, dtype=torch.float64, device=dev)
+    counter = torch.zeros(1, dtype=torch.int64, device=dev)
+    t = N.JobTable()
+    t.n_jobs = 1
+    j = t.jobs[0]
+    j.packed = mat.grid.voxels.data_ptr()
+    j.grid = mat.grid.desc
+    j.state = state.data_ptr()
+    j.order = order.data_ptr()
+    j.n_rays = n
+    j.seg_nsub = seg_nsub.data_ptr()
+    j.seg_h = seg_h.data_ptr()
+    j.n_seg = seg_nsub.numel()
+    j.sched_steps = int(nsub.sum())
+    j.step_begin = 0
+    j.step_end = int(nsub.sum())
+    j.out_state = out_state.data_ptr()
+    j.out_tau = out_tau.data_ptr()
+    j.out_path = out_path.data_ptr()
+    j.step_counter = counter.data_ptr()
+    N.check(L.slt_job_table_prepare(ctypes.byref(t)))
+    assert t.n_units == (n + 63) // 64
+    raw = torch.frombuffer(bytearray(ctypes.string_at(ctypes.addressof(t), ctypes.sizeof(t))),
+                           dtype=torch.uint8)
+    table = raw.to(dev)
+    N.check(L.slt_advance(table.data_ptr(), 8, N.stream_handle()))
+    torch.cuda.synchronize()
+    queues = table[: 9 * 128].cpu().view(torch.int32).numpy()
+    assert not queues.any(), queues[queues != 0]
+    ref = engine.trace(mat.grid, mat.rays, mat.save_times, step_fraction=0.5, save_all=False,
+                       want_path=True)
+    assert int(counter.item()) == int(ref.total_steps.item())
+    _close(out_state, ref.states[0])
+    _close(out_path, ref.path)
+    _close(out_tau, ref.tau[0])

@@ -71,8 +71,15 @@ def test_job_table_prepare_units_and_errors():
     t.n_jobs = 3
     for i, n in enumerate((1000, 256, 1)):
         t.jobs[i] = _valid_job(n)
+    t.next_unit[3][0] = 7
+    t.waves_done[0] = 9
     assert L.slt_job_table_prepare(ctypes.byref(t)) == N.SLT_OK
-    assert list(t.unit_begin[:4]) == [0, 4, 5, 6] and t.n_units == 6
+    # one unit = 64 rays (a wavefront); the work queues start empty
+    assert list(t.unit_begin[:4]) == [0, 16, 20, 21] and t.n_units == 21
+    assert t.next_unit[3][0] == 0 and t.waves_done[0] == 0
+    # the queues sit on their own 128-byte lines ahead of the job list
+    assert N.JobTable.next_unit.offset == 0 and N.JobTable.waves_done.offset == 1024
+    assert N.JobTable.jobs.offset % 8 == 0
     # a finishing batch without an output buffer is refused
     t.jobs[1].step_end = 100
     assert L.slt_job_table_prepare(ctypes.byref(t)) == N.SLT_ERR_INVALID
