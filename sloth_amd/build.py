@@ -17,11 +17,15 @@ LIB = os.path.join(LIBDIR, "libsloth_hip.so")
 SOURCES = ["abi.hip", "grid_pack.hip", "trace.hip", "detector.hip", "adjoint.hip"]
 ARCH = os.environ.get("SLOTH_OFFLOAD_ARCH", "gfx950")
 
-# Packed-FP32 VALU forms are disabled for all device code.
+# Packed-FP32 VALU forms are disabled for all device code (some of their forms
+# have returned wrong results on this hardware).  FMAs are formed only inside
+# one source expression (-ffp-contract=on), so the offline, serving and adjoint
+# kernels, which inline the same integration code into different control flow,
+# round identically and give bit-equal results.
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-fno-fast-math", "-ffp-contract=fast", "-Wall", "-Wno-unused-function",
+    "-fno-fast-math", "-ffp-contract=on", "-Wall", "-Wno-unused-function",
     "-munsafe-fp-atomics",
 ]
 

@@ -274,7 +274,10 @@ __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float x, fl
         ty = u1 - (float)i1;
         tz = u2 - (float)i2;
         id = (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2));
-        if (id != cc.id) load_cell(G, id, cc);
+        if (id != cc.id) {
+            load_cell(G, id, cc);
+            cc.id = id;
+        }
         // channel-contracted coefficients: Kq = sum_ch Ab_ch * kq_ch
         auto dot = [&](const float4& k) {
             return fmaf(Ab.w, k.w, fmaf(Ab.z, k.z, fmaf(Ab.y, k.y, Ab.x * k.x)));

@@ -8,11 +8,28 @@
 
 namespace slt {
 
-// Trilinear polynomial of the cached cell, one float4 (gx, gy, gz, kappa) per
+// Trilinear polynomial of a cell, one float4 (gx, gy, gz, kappa) per
 // coefficient.
-struct Cell {
-    int id;  // voxel index of corner (i, j, k); -1 = none cached
+struct CellPoly {
     float4 k0, kx, ky, kz, kxy, kxz, kyz, kxyz;
+};
+
+// Cell cached by the adjoint's reverse steps: polynomial + voxel index.
+struct Cell : CellPoly {
+    int id;  // voxel index of corner (i, j, k); -1 = none cached
+};
+
+// Cell cached by the forward integration: polynomial + the cell's lower
+// corner in grid units (i, j, k as floats; NaN = none cached) + whether the
+// cell touches a face of the grid.  A stage point whose local coordinates
+// u - corner all lie in [0, 1) is in the cached cell, so the common case needs
+// no voxel index; if the cell is interior the point is also a whole cell away
+// from every face and needs no box test.  Points in a face cell still take the
+// box test in physical coordinates, so a point an ulp outside a face is outside
+// whatever u rounds to.
+struct FwdCell : CellPoly {
+    float c0, c1, c2;
+    bool face;
 };
 
 __device__ __forceinline__ float4 f4sub(float4 a, float4 b) {
@@ -26,7 +43,7 @@ __device__ __forceinline__ float4 voxel(const SLT_GLOBAL slt_f4* __restrict__ p,
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
-__device__ __forceinline__ void load_cell(const DevGrid& G, int id, Cell& c) {
+__device__ __forceinline__ void load_cell(const DevGrid& G, int id, CellPoly& c) {
     const SLT_GLOBAL slt_f4* __restrict__ p =
         to_global(reinterpret_cast<const slt_f4*>(G.vox)) + id;
     const float4 c000 = voxel(p, 0);
@@ -47,7 +64,6 @@ __device__ __forceinline__ void load_cell(const DevGrid& G, int id, Cell& c) {
     c.kyz = f4sub(dz01, dz00);
     c.kxz = f4sub(dz10, dz00);
     c.kxyz = f4sub(f4sub(dz11, dz10), c.kyz);
-    c.id = id;
 }
 
 __device__ __forceinline__ float tri(float tx, float ty, float tz, float k0, float kx, float ky,
@@ -63,8 +79,67 @@ __device__ __forceinline__ bool in_box(const DevGrid& G, float x, float y, float
            (z <= G.hi2);
 }
 
+// (gx, gy, gz, kappa) of the cell polynomial at local coordinates t.
+template <bool KAPPA>
+__device__ __forceinline__ void poly_eval(const CellPoly& cc, float tx, float ty, float tz,
+                                          float4& out) {
+    out.x = tri(tx, ty, tz, cc.k0.x, cc.kx.x, cc.ky.x, cc.kz.x, cc.kxy.x, cc.kxz.x, cc.kyz.x,
+                cc.kxyz.x);
+    out.y = tri(tx, ty, tz, cc.k0.y, cc.kx.y, cc.ky.y, cc.kz.y, cc.kxy.y, cc.kxz.y, cc.kyz.y,
+                cc.kxyz.y);
+    out.z = tri(tx, ty, tz, cc.k0.z, cc.kx.z, cc.ky.z, cc.kz.z, cc.kxy.z, cc.kxz.z, cc.kyz.z,
+                cc.kxyz.z);
+    out.w = KAPPA ? tri(tx, ty, tz, cc.k0.w, cc.kx.w, cc.ky.w, cc.kz.w, cc.kxy.w, cc.kxz.w,
+                        cc.kyz.w, cc.kxyz.w)
+                  : 0.f;
+}
+
+// t in [0, 1) (false for -0.0, negatives and NaN): one unsigned compare of the
+// bits.  For t = u - corner this holds exactly when floor(u) == corner (the
+// subtraction is exact there), so the fast path picks the same cell as the
+// index computation and the result does not depend on which cell was cached
+// (a point on the upper face of the top cell takes the index path).
+__device__ __forceinline__ bool unit_interval(float t) {
+    return __float_as_uint(t) < 0x3f800000u;
+}
+
 // Interpolated (gx, gy, gz, kappa) at (x, y, z); zero and false outside the
 // closed box.  CHECK = false when the caller already knows the point is inside.
+template <bool KAPPA, bool CHECK>
+__device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, float z, FwdCell& cc,
+                                         float4& out) {
+    const float u0 = (x - G.lo0) * G.id0;
+    const float u1 = (y - G.lo1) * G.id1;
+    const float u2 = (z - G.lo2) * G.id2;
+    float tx = u0 - cc.c0;
+    float ty = u1 - cc.c1;
+    float tz = u2 - cc.c2;
+    if (!(unit_interval(tx) & unit_interval(ty) & unit_interval(tz))) {
+        if (CHECK && !in_box(G, x, y, z)) {
+            out = make_float4(0.f, 0.f, 0.f, 0.f);
+            return false;
+        }
+        const int i0 = min((int)u0, G.n0 - 2);  // u >= 0 inside the box: truncation == floor
+        const int i1 = min((int)u1, G.n1 - 2);
+        const int i2 = min((int)u2, G.n2 - 2);
+        cc.c0 = (float)i0;
+        cc.c1 = (float)i1;
+        cc.c2 = (float)i2;
+        cc.face = (min(min(i0, i1), i2) == 0) | (i0 == G.n0 - 2) | (i1 == G.n1 - 2) |
+                  (i2 == G.n2 - 2);
+        tx = u0 - cc.c0;
+        ty = u1 - cc.c1;
+        tz = u2 - cc.c2;
+        load_cell(G, (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2)), cc);
+    } else if (CHECK && cc.face && !in_box(G, x, y, z)) {
+        out = make_float4(0.f, 0.f, 0.f, 0.f);
+        return false;
+    }
+    poly_eval<KAPPA>(cc, tx, ty, tz, out);
+    return true;
+}
+
+// The same with the adjoint's cell cache (keyed by voxel index).
 template <bool KAPPA, bool CHECK>
 __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, float z, Cell& cc,
                                          float4& out) {
@@ -75,23 +150,15 @@ __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, flo
     const float u0 = (x - G.lo0) * G.id0;
     const float u1 = (y - G.lo1) * G.id1;
     const float u2 = (z - G.lo2) * G.id2;
-    const int i0 = min((int)u0, G.n0 - 2);  // u >= 0 inside the box: truncation == floor
+    const int i0 = min((int)u0, G.n0 - 2);
     const int i1 = min((int)u1, G.n1 - 2);
     const int i2 = min((int)u2, G.n2 - 2);
-    const float tx = u0 - (float)i0;
-    const float ty = u1 - (float)i1;
-    const float tz = u2 - (float)i2;
     const int id = (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2));
-    if (id != cc.id) load_cell(G, id, cc);
-    out.x = tri(tx, ty, tz, cc.k0.x, cc.kx.x, cc.ky.x, cc.kz.x, cc.kxy.x, cc.kxz.x, cc.kyz.x,
-                cc.kxyz.x);
-    out.y = tri(tx, ty, tz, cc.k0.y, cc.kx.y, cc.ky.y, cc.kz.y, cc.kxy.y, cc.kxz.y, cc.kyz.y,
-                cc.kxyz.y);
-    out.z = tri(tx, ty, tz, cc.k0.z, cc.kx.z, cc.ky.z, cc.kz.z, cc.kxy.z, cc.kxz.z, cc.kyz.z,
-                cc.kxyz.z);
-    out.w = KAPPA ? tri(tx, ty, tz, cc.k0.w, cc.kx.w, cc.ky.w, cc.kz.w, cc.kxy.w, cc.kxz.w,
-                        cc.kyz.w, cc.kxyz.w)
-                  : 0.f;
+    if (id != cc.id) {
+        load_cell(G, id, cc);
+        cc.id = id;
+    }
+    poly_eval<KAPPA>(cc, u0 - (float)i0, u1 - (float)i1, u2 - (float)i2, out);
     return true;
 }
 
@@ -189,8 +256,9 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                                           const float* __restrict__ seg_h, int nseg,
                                           int64_t g_begin, int64_t g_end, const SaveSink& sink,
                                           Tape& tape) {
-    Cell cc;
-    cc.id = -1;
+    FwdCell cc;
+    cc.c0 = cc.c1 = cc.c2 = __builtin_nanf("");
+    cc.face = true;
     float rbf[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) rbf[c] = (float)s.rb[c];
