@@ -166,6 +166,44 @@ int slt_histogram2d(const void* x, int64_t x_stride, const void* y, int64_t y_st
                     uint32_t* counts, double* wsum, slt_stream_t stream);
 
 /*
+ * k5: ray-transfer-matrix optics between the plasma and the detector.
+ * Replaces the element functions lens / sym_lens / travel /
+ * circular_aperture / circular_stop / rect_aperture / knife_edge
+ * (src/processing/diagnostics.py:28-128) as chained by the diagnostics'
+ * solve methods (:284-406), with m_to_mm (:16-20) in front.
+ *
+ * rays: 4 x n (x [m], theta, y [m], phi), fp32 (rays_f64 = 0) or fp64, row
+ * stride ld.  Rows 0 and 2 are multiplied by pos_scale in the input's
+ * precision (1e3 for the reference's m -> mm), then the n_ops elements are
+ * applied in order and the result is written as fp64 to out (4 x n, row
+ * stride out_ld; may not alias rays).  A linear element is the product of
+ * its 4x4 matrix with the ray, each component accumulated over the four
+ * inputs with fused multiply-adds (NaN rays stay NaN, as in the reference's
+ * matmul); a rejected ray becomes NaN in all four rows (r[:, filt] = None).
+ * Squares are taken as R*R.  ops is a host array (copied at launch).
+ */
+#define SLT_MAX_OPTICS 32
+enum {
+    SLT_OPTIC_TRAVEL = 0,            /* a = d: [[1, d], [0, 1]] on both axes  (:52-65)    */
+    SLT_OPTIC_LENS = 1,              /* a = f1 (x), b = f2 (y): [[1, 0], [-1/f, 1]] (:28-50) */
+    SLT_OPTIC_CIRCULAR_APERTURE = 2, /* a = R: reject x^2 + y^2 > R^2        (:67-75)    */
+    SLT_OPTIC_CIRCULAR_STOP = 3,     /* a = R: reject x^2 + y^2 < R^2        (:77-85)    */
+    SLT_OPTIC_RECT_APERTURE = 4,     /* a = Lx, b = Ly: reject x^2 > Lx^2 and y^2 > Ly^2 (:98-109) */
+    SLT_OPTIC_KNIFE_EDGE = 5         /* a = offset, b = direction (> 0: reject r[axis] > offset,
+                                        < 0: reject r[axis] < offset), axis 0 (x) or 2 (y) (:111-128) */
+};
+
+typedef struct slt_optic {
+    int32_t kind;
+    int32_t axis;
+    double a;
+    double b;
+} slt_optic;
+
+int slt_rtm(const void* rays, int32_t rays_f64, int64_t ld, int64_t n, double pos_scale,
+            const slt_optic* ops, int32_t n_ops, double* out, int64_t out_ld, slt_stream_t stream);
+
+/*
  * Morton (Z-order) keys of the rays' start positions in the plane transverse
  * to the probing axis, quantised to 16 bits per axis over [lo, hi] of the grid
  * (clamped).  Sorting rays by these keys keeps each wavefront's corner gathers

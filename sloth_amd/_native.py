@@ -22,11 +22,18 @@ EXPORTED = (
     "slt_kappa", "slt_trace", "slt_jones", "slt_histogram2d", "slt_morton_keys",
     "slt_ray_state_bytes", "slt_ray_state_init", "slt_job_table_prepare", "slt_advance",
     "slt_checkpoint_bytes", "slt_tape_bytes", "slt_forward_ckpt", "slt_adjoint",
-    "slt_adjoint_unpack",
+    "slt_adjoint_unpack", "slt_rtm",
 )
 
 SLT_MAX_JOBS = 64
 SLT_WORK_QUEUES = 8
+SLT_MAX_OPTICS = 32
+SLT_OPTIC_TRAVEL = 0
+SLT_OPTIC_LENS = 1
+SLT_OPTIC_CIRCULAR_APERTURE = 2
+SLT_OPTIC_CIRCULAR_STOP = 3
+SLT_OPTIC_RECT_APERTURE = 4
+SLT_OPTIC_KNIFE_EDGE = 5
 
 
 class GridDesc(ctypes.Structure):
@@ -101,6 +108,11 @@ class Job(ctypes.Structure):
     ]
 
 
+class Optic(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("axis", ctypes.c_int32),
+                ("a", ctypes.c_double), ("b", ctypes.c_double)]
+
+
 class JobTable(ctypes.Structure):
     _fields_ = [
         ("next_unit", (ctypes.c_uint32 * 32) * SLT_WORK_QUEUES),
@@ -143,6 +155,7 @@ def lib():
     L.slt_jones.argtypes = [vp, i64, i64, i32, vp, vp, vp, vp]
     L.slt_histogram2d.argtypes = [vp, i64, vp, i64, i32, i64, vp, vp, i32, vp, i32, vp, vp, vp, vp]
     L.slt_morton_keys.argtypes = [vp, i64, ctypes.POINTER(GridDesc), vp, vp]
+    L.slt_rtm.argtypes = [vp, i32, i64, i64, dbl, ctypes.POINTER(Optic), i32, vp, i64, vp]
     L.slt_ray_state_bytes.argtypes = [i64]
     L.slt_ray_state_bytes.restype = i64
     L.slt_ray_state_init.argtypes = [vp, i64, vp, vp, vp]

@@ -99,8 +99,8 @@ __device__ __forceinline__ void poly_eval(const CellPoly& cc, float tx, float ty
 // subtraction is exact there), so the fast path picks the same cell as the
 // index computation and the result does not depend on which cell was cached
 // (a point on the upper face of the top cell takes the index path).
-__device__ __forceinline__ bool unit_interval(float t) {
-    return __float_as_uint(t) < 0x3f800000u;
+__device__ __forceinline__ unsigned unit_interval(float t) {
+    return __float_as_uint(t) < 0x3f800000u ? 1u : 0u;
 }
 
 // Interpolated (gx, gy, gz, kappa) at (x, y, z); zero and false outside the
