@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(256) optics_kernel(OpticsK K) {
                 break;
             }
             case SLT_OPTIC_KNIFE_EDGE: {  // a = offset, b = direction sign
-                const double v = r[K.axis[k]];
+                const double v = K.axis[k] == 0 ? r[0] : r[2];
                 reject = b > 0.0 ? v > a : v < a;
                 break;
             }
@@ -133,11 +133,6 @@ extern "C" int slt_rtm(const void* rays, int32_t rays_f64, int64_t ld, int64_t n
     clear_error();
     if (n < 0 || n_ops < 0 || n_ops > SLT_MAX_OPTICS || (n_ops > 0 && !ops)) {
         set_error("slt_rtm: need n >= 0 and 0 <= n_ops <= %d", SLT_MAX_OPTICS);
-        return SLT_ERR_INVALID;
-    }
-    if (n == 0) return SLT_OK;
-    if (!rays || !out || ld < n || out_ld < n) {
-        set_error("slt_rtm: rays and out must be 4 x n with row strides >= n");
         return SLT_ERR_INVALID;
     }
     OpticsK K;
@@ -186,6 +181,11 @@ extern "C" int slt_rtm(const void* rays, int32_t rays_f64, int64_t ld, int64_t n
                 set_error("slt_rtm: element %d has unknown kind %d", k, o.kind);
                 return SLT_ERR_INVALID;
         }
+    }
+    if (n == 0) return SLT_OK;
+    if (!rays || !out || ld < n || out_ld < n) {
+        set_error("slt_rtm: rays and out must be 4 x n with row strides >= n");
+        return SLT_ERR_INVALID;
     }
     const int64_t blocks = (n + 255) / 256;
     if (blocks > INT32_MAX) {
