@@ -264,9 +264,9 @@ __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float x, fl
     int id = -1;
     float tx = 0.f, ty = 0.f, tz = 0.f;
     if (in) {
-        const float u0 = (x - G.lo0) * G.id0;
-        const float u1 = (y - G.lo1) * G.id1;
-        const float u2 = (z - G.lo2) * G.id2;
+        const float u0 = fmaf(x, G.id0, G.ul0);
+        const float u1 = fmaf(y, G.id1, G.ul1);
+        const float u2 = fmaf(z, G.id2, G.ul2);
         const int i0 = min((int)u0, G.n0 - 2);
         const int i1 = min((int)u1, G.n1 - 2);
         const int i2 = min((int)u2, G.n2 - 2);

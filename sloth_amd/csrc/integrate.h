@@ -108,9 +108,9 @@ __device__ __forceinline__ unsigned unit_interval(float t) {
 template <bool KAPPA, bool CHECK>
 __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, float z, FwdCell& cc,
                                          float4& out) {
-    const float u0 = (x - G.lo0) * G.id0;
-    const float u1 = (y - G.lo1) * G.id1;
-    const float u2 = (z - G.lo2) * G.id2;
+    const float u0 = fmaf(x, G.id0, G.ul0);
+    const float u1 = fmaf(y, G.id1, G.ul1);
+    const float u2 = fmaf(z, G.id2, G.ul2);
     float tx = u0 - cc.c0;
     float ty = u1 - cc.c1;
     float tz = u2 - cc.c2;
@@ -119,7 +119,9 @@ __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, flo
             out = make_float4(0.f, 0.f, 0.f, 0.f);
             return false;
         }
-        const int i0 = min((int)u0, G.n0 - 2);  // u >= 0 inside the box: truncation == floor
+        // inside the box u > -1 (a point on the lower face may round a hair below 0):
+        // truncation is floor for u >= 0 and gives cell 0 for the rest
+        const int i0 = min((int)u0, G.n0 - 2);
         const int i1 = min((int)u1, G.n1 - 2);
         const int i2 = min((int)u2, G.n2 - 2);
         cc.c0 = (float)i0;
@@ -147,9 +149,9 @@ __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, flo
         out = make_float4(0.f, 0.f, 0.f, 0.f);
         return false;
     }
-    const float u0 = (x - G.lo0) * G.id0;
-    const float u1 = (y - G.lo1) * G.id1;
-    const float u2 = (z - G.lo2) * G.id2;
+    const float u0 = fmaf(x, G.id0, G.ul0);
+    const float u1 = fmaf(y, G.id1, G.ul1);
+    const float u2 = fmaf(z, G.id2, G.ul2);
     const int i0 = min((int)u0, G.n0 - 2);
     const int i1 = min((int)u1, G.n1 - 2);
     const int i2 = min((int)u2, G.n2 - 2);
@@ -280,7 +282,10 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                     rh[c] = rbf[c] + s.off[c];
                     vh[c] = s.v0[c] + s.dv[c];
                 }
-                const bool in1 = in_box(G, rh[0], rh[1], rh[2]);
+                // The first stage doubles as the box test of the step's start point:
+                // zero RHS and false outside the box.
+                float4 A1, A2, A3, A4;
+                const bool in1 = eval_rhs<KAPPA, true>(G, rh[0], rh[1], rh[2], cc, A1);
                 if (!in1) {
                     const int m = ballistic_steps(G, rh, vh, h, kend - k);
                     if (m > 0) {
@@ -301,9 +306,7 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                 }
                 // One RK4 step.
                 tape.rk(rh, vh, h);
-                float4 A1 = make_float4(0.f, 0.f, 0.f, 0.f), A2, A3, A4;
                 bool in = in1;
-                if (in1) eval_rhs<KAPPA, false>(G, rh[0], rh[1], rh[2], cc, A1);
                 const float v2x = fmaf(hh, A1.x, vh[0]), v2y = fmaf(hh, A1.y, vh[1]),
                             v2z = fmaf(hh, A1.z, vh[2]);
                 in |= eval_rhs<KAPPA, true>(G, fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]),

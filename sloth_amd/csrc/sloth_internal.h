@@ -25,6 +25,7 @@ struct DevGrid {
     float lo0, lo1, lo2;
     float hi0, hi1, hi2;
     float id0, id1, id2;
+    float ul0, ul1, ul2;  // -lo * inv_d: the grid coordinate is u = fma(x, inv_d, ul)
     int nchan;
 };
 
@@ -51,6 +52,7 @@ inline int make_dev_grid(const void* packed, const slt_grid_desc* g, DevGrid* d)
     d->lo0 = g->lo[0]; d->lo1 = g->lo[1]; d->lo2 = g->lo[2];
     d->hi0 = g->hi[0]; d->hi1 = g->hi[1]; d->hi2 = g->hi[2];
     d->id0 = g->inv_d[0]; d->id1 = g->inv_d[1]; d->id2 = g->inv_d[2];
+    d->ul0 = -(d->lo0 * d->id0); d->ul1 = -(d->lo1 * d->id1); d->ul2 = -(d->lo2 * d->id2);
     d->nchan = g->nchan;
     return SLT_OK;
 }
@@ -65,6 +67,7 @@ __host__ __device__ inline void make_dev_grid_device(const void* packed, const s
     d->lo0 = g->lo[0]; d->lo1 = g->lo[1]; d->lo2 = g->lo[2];
     d->hi0 = g->hi[0]; d->hi1 = g->hi[1]; d->hi2 = g->hi[2];
     d->id0 = g->inv_d[0]; d->id1 = g->inv_d[1]; d->id2 = g->inv_d[2];
+    d->ul0 = -(d->lo0 * d->id0); d->ul1 = -(d->lo1 * d->id1); d->ul2 = -(d->lo2 * d->id2);
     d->nchan = g->nchan;
 }
 
