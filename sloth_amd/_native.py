@@ -141,6 +141,10 @@ def lib():
         raise ImportError(
             f"{LIB_PATH} is missing: build the HIP library first "
             "(python -m sloth_amd.build, or __graft_entry__.build())")
+    # torch first: the library then binds to the HIP runtime torch has loaded,
+    # so both share one runtime (and one device context) in the process.
+    import torch  # noqa: F401
+
     L = ctypes.CDLL(LIB_PATH)
     vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
     L.slt_version.restype = ctypes.c_int

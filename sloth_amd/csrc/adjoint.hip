@@ -200,41 +200,96 @@ __device__ __forceinline__ void scatter_corners(const DevGrid& G, float* __restr
     f4atomic(p + 4 * (G.s0 + G.s1 + G.s2), c111);
 }
 
+// Where flushed moments go: the voxel adjoint in global memory and this
+// wavefront's flush slots in LDS (kFlushSlots cells x 32 floats).
+constexpr int kFlushSlots = 16;
+#define SLT_LDS __attribute__((address_space(3)))
+
+struct AdjOut {
+    float* __restrict__ adj;
+    SLT_LDS float* lds;
+};
+
+__device__ __forceinline__ void lds_add4(SLT_LDS float* p, const float4& v) {
+    __hip_atomic_fetch_add(p + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __hip_atomic_fetch_add(p + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __hip_atomic_fetch_add(p + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __hip_atomic_fetch_add(p + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// Reads a float4 of a slot and leaves zeros behind.
+__device__ __forceinline__ float4 lds_take4(SLT_LDS float* p) {
+    const float4 v = make_float4(p[0], p[1], p[2], p[3]);
+    p[0] = p[1] = p[2] = p[3] = 0.f;
+    return v;
+}
+
+// Orders this wavefront's LDS accesses across its lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Collective flush (whole wavefront, uniform control flow): the lanes with
-// `need` set are grouped by cell; each group's moments are summed across the
-// wavefront and lane 63 scatters the group's corner adjoints with atomics.
-// Lanes from one sorted wavefront share a few cells, so this replaces 64
-// lanes' atomics on the same voxels with one lane's.
-__device__ __forceinline__ void flush_wave(const DevGrid& G, float* __restrict__ adj, Moments& M,
+// `need` set give their moments to their cell.  A sorted wavefront holds a few
+// cells at a time, and at a flush nearly all its lanes leave theirs together,
+// so each distinct cell gets an LDS slot (the first lane still pending leads
+// it), every lane adds its moments to its slot with LDS atomics, and each
+// slot's leader reads the sum back and scatters the cell's corner adjoints
+// with global atomics: one lane's atomics per cell, with no cross-lane
+// reduction per cell.  Lanes of cells beyond the slots scatter their own.
+__device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, Moments& M,
                                            bool need) {
     uint64_t pending = __ballot(need);
     const int lane = __lane_id();
-    while (pending) {
+    int slot = -1;  // the slot this lane adds to
+    int led = -1;   // the slot this lane reads back and scatters
+    for (int g = 0; pending && g < kFlushSlots; ++g) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const int key = __builtin_amdgcn_readlane(M.id, leader);
         const bool mine = need && M.id == key;
         pending &= ~__ballot(mine);
-        Moments S;
-        S.m0 = wave_sum63(M.m0, mine);
-        S.mx = wave_sum63(M.mx, mine);
-        S.my = wave_sum63(M.my, mine);
-        S.mz = wave_sum63(M.mz, mine);
-        S.mxy = wave_sum63(M.mxy, mine);
-        S.mxz = wave_sum63(M.mxz, mine);
-        S.myz = wave_sum63(M.myz, mine);
-        S.mxyz = wave_sum63(M.mxyz, mine);
-        if (lane == 63) scatter_corners(G, adj, key, S);
+        if (mine) slot = g;
+        if (lane == leader) led = g;
     }
+    if (need && slot < 0) scatter_corners(G, out.adj, M.id, M);
+    if (slot >= 0) {
+        SLT_LDS float* p = out.lds + slot * 32;
+        lds_add4(p + 0, M.m0);
+        lds_add4(p + 4, M.mx);
+        lds_add4(p + 8, M.my);
+        lds_add4(p + 12, M.mz);
+        lds_add4(p + 16, M.mxy);
+        lds_add4(p + 20, M.mxz);
+        lds_add4(p + 24, M.myz);
+        lds_add4(p + 28, M.mxyz);
+    }
+    wave_lds_sync();
+    if (led >= 0) {
+        SLT_LDS float* p = out.lds + led * 32;
+        Moments S;
+        S.m0 = lds_take4(p + 0);
+        S.mx = lds_take4(p + 4);
+        S.my = lds_take4(p + 8);
+        S.mz = lds_take4(p + 12);
+        S.mxy = lds_take4(p + 16);
+        S.mxz = lds_take4(p + 20);
+        S.myz = lds_take4(p + 24);
+        S.mxyz = lds_take4(p + 28);
+        scatter_corners(G, out.adj, M.id, S);
+    }
+    wave_lds_sync();
 }
 
 // Adds one stage's adjoint Ab at cell `id`, local coordinates t, to the lane's
 // moments (contrib = this lane has a contribution).  Called by every lane of
 // the wavefront together.
-__device__ __forceinline__ void accumulate(const DevGrid& G, float* __restrict__ adj, Moments& M,
+__device__ __forceinline__ void accumulate(const DevGrid& G, const AdjOut& out, Moments& M,
                                            bool contrib, int id, float tx, float ty, float tz,
                                            const float4& Ab) {
     const bool move = contrib && id != M.id;
-    if (__ballot(move && M.id >= 0)) flush_wave(G, adj, M, move && M.id >= 0);
+    if (__ballot(move && M.id >= 0)) flush_wave(G, out, M, move && M.id >= 0);
     if (move) {
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = z4;
@@ -258,7 +313,7 @@ __device__ __forceinline__ void accumulate(const DevGrid& G, float* __restrict__
 // wavefront calls it together; `act` lanes contribute.
 __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float x, float y, float z,
                                          const float4& Ab, Cell& cc, Moments& M,
-                                         float* __restrict__ adj, float rb[3]) {
+                                         const AdjOut& out, float rb[3]) {
     rb[0] = rb[1] = rb[2] = 0.f;
     const bool in = act && in_box(G, x, y, z);
     int id = -1;
@@ -289,7 +344,7 @@ __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float x, fl
         rb[1] = G.id1 * fmaf(tz, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxy, Ky));
         rb[2] = G.id2 * fmaf(ty, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxz, Kz));
     }
-    accumulate(G, adj, M, in, id, tx, ty, tz, Ab);
+    accumulate(G, out, M, in, id, tx, ty, tz, Ab);
 }
 
 // Transposed RK4 step.  On entry rb, vb are the adjoints of the state after the
@@ -299,7 +354,7 @@ template <bool KAPPA>
 __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float rh[3],
                                          const float vh[3], float h, float rb[3], float vb[3],
                                          float tb, Cell& cc, Moments& M,
-                                         float* __restrict__ adj) {
+                                         const AdjOut& out) {
     const float hh = 0.5f * h;
     const float h6 = h * (1.0f / 6.0f);
     const float h3 = h * (1.0f / 3.0f);
@@ -320,23 +375,23 @@ __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float
     float p4[3], p3[3], p2[3], p1[3];
     // stage 4: a4 enters v' with weight h/6
     const float4 B4 = make_float4(h6 * vb[0], h6 * vb[1], h6 * vb[2], h6 * tk);
-    eval_adj(G, act, r4[0], r4[1], r4[2], B4, cc, M, adj, p4);
+    eval_adj(G, act, r4[0], r4[1], r4[2], B4, cc, M, out, p4);
     const float v3b[3] = {h * p4[0], h * p4[1], h * p4[2]};  // r4 = r + h v3
     // stage 3: a3 enters r' (h^2/6) and v' (h/3)
     const float4 B3 = make_float4(fmaf(hh6, rb[0], h3 * vb[0]), fmaf(hh6, rb[1], h3 * vb[1]),
                                   fmaf(hh6, rb[2], h3 * vb[2]), h3 * tk);
-    eval_adj(G, act, r3[0], r3[1], r3[2], B3, cc, M, adj, p3);
+    eval_adj(G, act, r3[0], r3[1], r3[2], B3, cc, M, out, p3);
     const float v2b[3] = {hh * p3[0], hh * p3[1], hh * p3[2]};  // r3 = r + h/2 v2
     // stage 2: a2 enters r' (h^2/6), v' (h/3) and v3 = v + h/2 a2
     const float4 B2 = make_float4(fmaf(hh, v3b[0], fmaf(hh6, rb[0], h3 * vb[0])),
                                   fmaf(hh, v3b[1], fmaf(hh6, rb[1], h3 * vb[1])),
                                   fmaf(hh, v3b[2], fmaf(hh6, rb[2], h3 * vb[2])), h3 * tk);
-    eval_adj(G, act, r2[0], r2[1], r2[2], B2, cc, M, adj, p2);
+    eval_adj(G, act, r2[0], r2[1], r2[2], B2, cc, M, out, p2);
     // stage 1: a1 enters r' (h^2/6), v' (h/6) and v2 = v + h/2 a1
     const float4 B1 = make_float4(fmaf(hh, v2b[0], fmaf(hh6, rb[0], h6 * vb[0])),
                                   fmaf(hh, v2b[1], fmaf(hh6, rb[1], h6 * vb[1])),
                                   fmaf(hh, v2b[2], fmaf(hh6, rb[2], h6 * vb[2])), h6 * tk);
-    eval_adj(G, act, rh[0], rh[1], rh[2], B1, cc, M, adj, p1);
+    eval_adj(G, act, rh[0], rh[1], rh[2], B1, cc, M, out, p1);
     if (act) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -359,6 +414,12 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
     const int64_t n = A.n;
     const int64_t tt = valid ? t : 0;  // in-bounds row for the lanes past the end
     const int64_t ray = valid ? (A.order ? (int64_t)A.order[t] : t) : 0;
+    // this wavefront's flush slots, zero between flushes
+    __shared__ float s_flush[(256 / 64) * kFlushSlots * 32];
+    const AdjOut out{A.grid_adj,
+                     (SLT_LDS float*)s_flush + (threadIdx.x / 64) * (kFlushSlots * 32)};
+    for (int i = __lane_id(); i < kFlushSlots * 32; i += 64) out.lds[i] = 0.f;
+    wave_lds_sync();
     float rb[3] = {0.f, 0.f, 0.f}, vb[3] = {0.f, 0.f, 0.f}, v0[3] = {0.f, 0.f, 0.f};
     float tb = 0.f;
     if (valid) {
@@ -414,10 +475,10 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
                 for (int a = 0; a < 3; ++a) vb[a] = fmaf(-hT, rb[a], vb[a]);
             }
             step_adj<KAPPA>(A.G, act && hT > 0.f, rh, vh, hT > 0.f ? hT : 0.f, rb, vb, tb, cc,
-                            M, A.grid_adj);
+                            M, out);
         }
     }
-    flush_wave(A.G, A.grid_adj, M, M.id >= 0);
+    flush_wave(A.G, out, M, M.id >= 0);
 }
 
 // ---------------------------------------------------------------------------
