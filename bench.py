@@ -331,6 +331,7 @@ def main():
                    "Te_eV": 20, "Z": 1, "coulomb_log": 10, "integrator": "rk4, 0.5 cell",
                    "horizon": "sqrt(8) L / c", "ray_steps_per_pass": total_steps,
                    "accumulators": "fp64 tau and path"},
+        "rhs_evals_per_s": 4.0 * value,   # RK4: 4 RHS evaluations per ray-step
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,

@@ -156,7 +156,7 @@ int slt_jones(const float* state, int64_t ld, int64_t n, int32_t probing_axis, f
  * anything else outside, and NaN, is dropped (numpy's rule).  Values are
  * compared in fp64 against the fp64 edge arrays (device, nxb+1 / nyb+1 entries).
  * x, y are device arrays of fp32 (is_f64 = 0) or fp64 (is_f64 = 1) with the
- * given element strides -- e.g. rows 0 and 2 of rf.
+ * given element strides (>= 0; 0 repeats one element) -- e.g. rows 0 and 2 of rf.
  * counts (uint32) and/or wsum (fp64, needs weights) are accumulated into, so
  * the caller zeroes them first.  mask: optional device uint8 per ray (0 = skip).
  */

@@ -94,7 +94,7 @@ extern "C" int slt_histogram2d(const void* x, int64_t x_stride, const void* y,
                                double* wsum, slt_stream_t stream) {
     clear_error();
     if (n < 0 || nxb < 1 || nyb < 1 || !xedges || !yedges || (!counts && !wsum) ||
-        (wsum && !weights) || (n > 0 && (!x || !y)) || x_stride < 1 || y_stride < 1) {
+        (n > 0 && (!x || !y || (wsum && !weights))) || x_stride < 0 || y_stride < 0) {
         set_error("slt_histogram2d: invalid arguments");
         return SLT_ERR_INVALID;
     }

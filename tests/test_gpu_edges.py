@@ -1,0 +1,87 @@
+"""Edge cases on the GPU: empty and degenerate inputs through each entry point."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _dev():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def test_histogram_empty_all_nan_and_all_outside():
+    import torch
+
+    from sloth_amd.processing.diagnostics import histogram2d_device
+
+    e = np.linspace(-1.0, 1.0, 9)
+    for case, x in (("empty", np.zeros(0)), ("nan", np.full(50, np.nan)),
+                    ("outside", np.full(50, 3.0))):
+        t = torch.from_numpy(x).cuda()
+        counts, wsum = histogram2d_device(t, t, e, e, weights=torch.ones(x.size, dtype=torch.float64))
+        assert counts.sum().item() == 0 and wsum.abs().sum().item() == 0.0, case
+    # the last edge belongs to the last bin, the first to the first (numpy's rule)
+    x = torch.tensor([-1.0, 1.0, 1.0 + 1e-12], dtype=torch.float64).cuda()
+    counts, _ = histogram2d_device(x, x, e, e)
+    c = counts.cpu().numpy()
+    assert c[0, 0] == 1 and c[-1, -1] == 1 and c.sum() == 2
+
+
+def test_optics_empty_and_all_rejected():
+    from sloth_amd.processing import diagnostics as D
+
+    d = D.Shadowgraphy(np.zeros((4, 0), np.float32))
+    d.two_lens_solve()
+    assert d.rf.shape == (4, 0)
+    d.histogram(pix_x=10, pix_y=10)
+    assert d.H.sum() == 0
+    far = np.zeros((4, 5), np.float32)
+    far[0] = 1.0                                  # 1 m off axis: outside every aperture
+    d = D.Schlieren(far)
+    d.DF_solve()
+    assert np.isnan(d.rf).all()
+    d.histogram(pix_x=10, pix_y=10)
+    assert d.H.sum() == 0
+
+
+def test_serving_single_ray_and_empty_requests():
+    from sloth_amd.serving import BeamSpec, FieldSpec, RequestSpec, ServingEngine, run_offline
+
+    reqs = [RequestSpec(rid=0, field=FieldSpec(kind="sine", side=32, length=6e-3),
+                        beam=BeamSpec(n_rays=1, half_width=1e-3, seed=1), bins=16),
+            RequestSpec(rid=1, field=FieldSpec(kind="kolmogorov", side=32, length=6e-3, seed=2),
+                        beam=BeamSpec(n_rays=0, half_width=1e-3, seed=2), bins=16),
+            RequestSpec(rid=2, field=FieldSpec(kind="kolmogorov", side=40, length=6e-3, seed=3),
+                        beam=BeamSpec(n_rays=3000, half_width=2e-3, seed=3), bins=16)]
+    off, _ = run_offline(reqs)
+    resp, stats = ServingEngine(max_slots=4, blocks=64, batch_work=1e5).run(reqs, realtime=False)
+    assert stats["ok"] == 3
+    assert resp[1].n_rays == 0 and resp[1].ray_steps == 0 and resp[1].image.sum().item() == 0
+    assert resp[0].n_rays == 1 and resp[0].ray_steps == off[0]["steps"] > 0
+    np.testing.assert_array_equal(resp[2].rf.cpu().numpy(), off[2]["rf"].cpu().numpy())
+    np.testing.assert_array_equal(resp[0].rf.cpu().numpy(), off[0]["rf"].cpu().numpy())
+
+
+def test_trace_rays_that_miss_the_grid():
+    """Rays that never touch the box move in straight lines and count no steps."""
+    from sloth_amd.core import engine
+    from sloth_amd.fields import sine_field
+
+    L = 6e-3
+    ne, coords = sine_field(24, L)
+    grid = engine.PackedGrid(ne, *coords, 2 * np.pi * 299792458.0 / 1064e-9)
+    s0 = np.zeros((6, 3), np.float32)
+    s0[0] = [5e-3, -4e-3, 0.0]        # two beside the box, one heading away from it
+    s0[1] = [0.0, 0.0, 0.0]
+    s0[2] = [-L / 2, -L / 2, -L]
+    s0[5] = [299792458.0, 299792458.0, -299792458.0]
+    t = np.array([0.0, 2 * L / 299792458.0])
+    r = engine.trace(grid, s0, t, save_all=False, want_path=True)
+    assert r.steps.cpu().numpy().tolist() == [0, 0, 0]
+    end = r.states[0].cpu().numpy()
+    np.testing.assert_allclose(end[2], s0[2] + s0[5] * t[-1], rtol=1e-6)
+    np.testing.assert_allclose(r.path.cpu().numpy(), 2 * L, rtol=1e-6)
