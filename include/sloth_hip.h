@@ -213,6 +213,22 @@ int slt_rtm(const void* rays, int32_t rays_f64, int64_t ld, int64_t n, double po
 int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int64_t* keys,
                     slt_stream_t stream);
 
+/*
+ * k6: probe beam made on the device (opt-in; no reference counterpart: the
+ * reference draws rays on the host from numpy's RandomState,
+ * src/core/beam.py:57-346, which the drop-in Beam keeps).  Ray i takes its
+ * random numbers from Philox4x32-10 with counter (i, word) and key `seed`, so
+ * each ray depends only on (seed, i).  Distributions as the reference's beam
+ * types: circular (radius size_a, uniform on the disc), square (half-width
+ * size_a), rectangular (half-widths size_a, size_b), linear (half-length size_a,
+ * x-z plane); divergence is the standard deviation of the angle to the probing
+ * axis; rays start at -ne_extent on the probing axis.  s0: 6 x n fp32 device.
+ */
+enum { SLT_BEAM_CIRCULAR = 0, SLT_BEAM_SQUARE = 1, SLT_BEAM_RECTANGULAR = 2, SLT_BEAM_LINEAR = 3 };
+
+int slt_beam(int64_t n, int32_t beam_type, int32_t probing_axis, double size_a, double size_b,
+             double divergence, double ne_extent, uint64_t seed, float* s0, slt_stream_t stream);
+
 /* ------------------------------------------------------------------------- */
 /* Serving: resumable ray state and multi-request step batches.              */
 /* No reference counterpart (the reference traces one request per call).     */

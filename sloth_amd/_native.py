@@ -22,7 +22,7 @@ EXPORTED = (
     "slt_kappa", "slt_trace", "slt_jones", "slt_histogram2d", "slt_morton_keys",
     "slt_ray_state_bytes", "slt_ray_state_init", "slt_job_table_prepare", "slt_advance",
     "slt_checkpoint_bytes", "slt_tape_bytes", "slt_forward_ckpt", "slt_adjoint",
-    "slt_adjoint_unpack", "slt_rtm",
+    "slt_adjoint_unpack", "slt_rtm", "slt_beam",
 )
 
 SLT_MAX_JOBS = 64
@@ -34,6 +34,7 @@ SLT_OPTIC_CIRCULAR_APERTURE = 2
 SLT_OPTIC_CIRCULAR_STOP = 3
 SLT_OPTIC_RECT_APERTURE = 4
 SLT_OPTIC_KNIFE_EDGE = 5
+SLT_BEAM_CIRCULAR, SLT_BEAM_SQUARE, SLT_BEAM_RECTANGULAR, SLT_BEAM_LINEAR = 0, 1, 2, 3
 
 
 class GridDesc(ctypes.Structure):
@@ -160,6 +161,7 @@ def lib():
     L.slt_histogram2d.argtypes = [vp, i64, vp, i64, i32, i64, vp, vp, i32, vp, i32, vp, vp, vp, vp]
     L.slt_morton_keys.argtypes = [vp, i64, ctypes.POINTER(GridDesc), vp, vp]
     L.slt_rtm.argtypes = [vp, i32, i64, i64, dbl, ctypes.POINTER(Optic), i32, vp, i64, vp]
+    L.slt_beam.argtypes = [i64, i32, i32, dbl, dbl, dbl, dbl, ctypes.c_uint64, vp, vp]
     L.slt_ray_state_bytes.argtypes = [i64]
     L.slt_ray_state_bytes.restype = i64
     L.slt_ray_state_init.argtypes = [vp, i64, vp, vp, vp]

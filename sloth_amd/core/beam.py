@@ -6,17 +6,52 @@ same draw order per beam type, so a seeded beam reproduces the reference's
 ray set (up to the reference computing the trigonometry in float32).  Ray
 synthesis stays on the host in NumPy to keep that RNG stream; it is input
 preparation, not part of the traced path.
+
+``device_rng=True`` (an extension) makes the rays on the GPU instead, with the
+k6 kernel's counter-based generator: same distributions, a different (but
+seeded and launch-independent) stream, and ``s0`` a device tensor.
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 
 from ..constants import C_LIGHT
 
+_BEAM_TYPES = {"circular": 0, "square": 1, "rectangular": 2, "rect_trackers": 2, "linear": 3}
+
+
+def device_beam(n, beam_size, divergence, ne_extent, *, probing_direction="z",
+                beam_type="circular", seed=42, device=None, stream=None):
+    """(6, n) float32 rays made on the GPU by ``slt_beam`` (Philox4x32-10 per ray)."""
+    import torch
+
+    from .. import _native as N
+
+    if beam_type not in _BEAM_TYPES:
+        raise ValueError("beam_type unrecognised! Accepted: circular, square, rectangular, "
+                         "linear, rect_trackers.")
+    if beam_type in ("rectangular", "rect_trackers"):
+        if len(beam_size) != 2:
+            raise AssertionError("Must pass a list of length 2 to initialise a rectangular beam.")
+        a, b = float(beam_size[0]), float(beam_size[1])
+    else:
+        if not np.isscalar(beam_size):
+            raise AssertionError("Received a sized beam_size, expected a float.")
+        a = b = float(beam_size)
+    axis = {"x": 0, "y": 1, "z": 2}.get(probing_direction, 1)
+    dev = device if device is not None else N.require_device()
+    s0 = torch.empty((6, int(n)), dtype=torch.float32, device=dev)
+    N.check(N.lib().slt_beam(int(n), _BEAM_TYPES[beam_type], axis, a, b, float(divergence),
+                             float(ne_extent), int(seed) & 0xFFFFFFFFFFFFFFFF, s0.data_ptr(),
+                             N.stream_handle(stream)))
+    return s0
+
 
 class Beam:
     def __init__(self, Np, beam_size, divergence, ne_extent, *, probing_direction="z",
-                 beam_type="circular", seeded=False):
+                 beam_type="circular", seeded=False, device_rng=False):
         self.Np = np.int64(Np)
         self.beam_size = beam_size
         self.divergence = divergence
@@ -24,6 +59,14 @@ class Beam:
         self.probing_direction = probing_direction
         self.beam_type = beam_type
         self.seed = 42 if seeded else None
+        if device_rng:
+            if beam_type == "even":
+                raise NotImplementedError("beam_type='even' is not functional in the reference")
+            seed = self.seed if seeded else int.from_bytes(os.urandom(8), "little")
+            self.s0 = device_beam(self.Np, beam_size, divergence, ne_extent,
+                                  probing_direction=probing_direction, beam_type=beam_type,
+                                  seed=seed)
+            return
         self.init_beam(-self.ne_extent, self.seed)
 
     def init_beam(self, ne_extent, seed):

@@ -3,8 +3,8 @@
 A request names its density field (kind, grid side, box length, seed), its
 beam (shape, half-width, ray count, seed), the wavelength and the absorption
 settings.  ``materialize`` builds the packed grid and the rays directly in HBM
-(rocFFT through torch for the Kolmogorov field, torch's device generator for
-the beam), so the host never touches grid-sized data.  The same spec always
+(rocFFT through torch for the Kolmogorov field, the k6 counter-based beam
+kernel for the rays), so the host never touches grid-sized data.  The same spec always
 produces the same inputs, which is what lets a request be compared alone and
 inside a serving mix.
 """
@@ -18,6 +18,7 @@ import torch
 
 from ..constants import C_LIGHT
 from ..core import engine
+from ..core.beam import device_beam
 
 
 @dataclass(frozen=True)
@@ -99,15 +100,10 @@ def field_on_device(spec: FieldSpec, device) -> torch.Tensor:
 
 
 def beam_on_device(spec: BeamSpec, length: float, device) -> torch.Tensor:
-    """(6, N) collimated square beam entering the z = -L/2 face, built on the device."""
-    gen = torch.Generator(device=device)
-    gen.manual_seed(int(spec.seed) + 7919)
-    n = spec.n_rays
-    s0 = torch.zeros((6, n), dtype=torch.float32, device=device)
-    s0[0:2] = (2 * torch.rand((2, n), generator=gen, device=device) - 1) * spec.half_width
-    s0[2] = float(np.float32(-length / 2))
-    s0[5] = C_LIGHT
-    return s0
+    """(6, N) collimated square beam entering the z = -L/2 face, made by the k6 kernel."""
+    return device_beam(spec.n_rays, spec.half_width, 0.0, float(np.float32(length / 2)),
+                       probing_direction="z", beam_type="square", seed=int(spec.seed) + 7919,
+                       device=device)
 
 
 @dataclass
