@@ -262,7 +262,7 @@ def main():
         if events is not None:
             events[1].record(stream)
         rf, amp = jones_device(res.states[0], "z", res.tau[0])
-        counts, wsum = histogram2d_device(rf[0], rf[2], edges, edges, weights=amp)
+        counts, wsum = histogram2d_device(rf[0], rf[2], edges, edges, weights=amp, order=order)
         return res, counts, wsum
 
     log("[bench] warmup ...")

@@ -159,11 +159,17 @@ int slt_jones(const float* state, int64_t ld, int64_t n, int32_t probing_axis, f
  * given element strides (>= 0; 0 repeats one element) -- e.g. rows 0 and 2 of rf.
  * counts (uint32) and/or wsum (fp64, needs weights) are accumulated into, so
  * the caller zeroes them first.  mask: optional device uint8 per ray (0 = skip).
+ * order: optional device int32 permutation of [0, n) (the Morton order the
+ * rays were traced in); rays are then visited as order[0..n) and a
+ * wavefront's entries for one cell are combined into one atomic.  Counts do
+ * not depend on it; wsum only through fp64 summation order.  Entries outside
+ * [0, n) are skipped.  nxb * nyb must fit in int32.
  */
 int slt_histogram2d(const void* x, int64_t x_stride, const void* y, int64_t y_stride,
-                    int32_t is_f64, int64_t n, const uint8_t* mask, const double* xedges,
-                    int32_t nxb, const double* yedges, int32_t nyb, const double* weights,
-                    uint32_t* counts, double* wsum, slt_stream_t stream);
+                    int32_t is_f64, int64_t n, const int32_t* order, const uint8_t* mask,
+                    const double* xedges, int32_t nxb, const double* yedges, int32_t nyb,
+                    const double* weights, uint32_t* counts, double* wsum,
+                    slt_stream_t stream);
 
 /*
  * k5: ray-transfer-matrix optics between the plasma and the detector.
@@ -205,12 +211,13 @@ int slt_rtm(const void* rays, int32_t rays_f64, int64_t ld, int64_t n, double po
 
 /*
  * Morton (Z-order) keys of the rays' start positions in the plane transverse
- * to the probing axis, quantised to 16 bits per axis over [lo, hi] of the grid
- * (clamped).  Sorting rays by these keys keeps each wavefront's corner gathers
+ * to the probing axis, quantised to 15 bits per axis over [lo, hi] of the grid
+ * (clamped) and interleaved into a non-negative 30-bit int32 (4 radix passes
+ * to sort).  Sorting rays by these keys keeps each wavefront's corner gathers
  * on a few cache lines.  No reference counterpart (the reference vmaps rays in
  * input order, src/core/propagator.py:792).
  */
-int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int64_t* keys,
+int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int32_t* keys,
                     slt_stream_t stream);
 
 /*

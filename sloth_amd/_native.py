@@ -158,7 +158,8 @@ def lib():
     L.slt_kappa.argtypes = [vp, i64, vp, i64, vp, i64, dbl, dbl, dbl, dbl, dbl, vp, vp]
     L.slt_trace.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(TraceArgs), vp]
     L.slt_jones.argtypes = [vp, i64, i64, i32, vp, vp, vp, vp]
-    L.slt_histogram2d.argtypes = [vp, i64, vp, i64, i32, i64, vp, vp, i32, vp, i32, vp, vp, vp, vp]
+    L.slt_histogram2d.argtypes = [vp, i64, vp, i64, i32, i64, vp, vp, vp, i32, vp, i32, vp, vp, vp,
+                                  vp]
     L.slt_morton_keys.argtypes = [vp, i64, ctypes.POINTER(GridDesc), vp, vp]
     L.slt_rtm.argtypes = [vp, i32, i64, i64, dbl, ctypes.POINTER(Optic), i32, vp, i64, vp]
     L.slt_beam.argtypes = [i64, i32, i32, dbl, dbl, dbl, dbl, ctypes.c_uint64, vp, vp]

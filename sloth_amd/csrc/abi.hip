@@ -28,9 +28,9 @@ int check_launch(const char* what) {
     return SLT_OK;
 }
 
-// Spreads the low 16 bits of v to the even bit positions of a 32-bit word.
-__device__ inline uint32_t spread16(uint32_t v) {
-    v &= 0xFFFFu;
+// Spreads the low 15 bits of v to the even bit positions of a 30-bit word.
+__device__ inline uint32_t spread15(uint32_t v) {
+    v &= 0x7FFFu;
     v = (v | (v << 8)) & 0x00FF00FFu;
     v = (v | (v << 4)) & 0x0F0F0F0Fu;
     v = (v | (v << 2)) & 0x33333333u;
@@ -38,21 +38,21 @@ __device__ inline uint32_t spread16(uint32_t v) {
     return v;
 }
 
-__device__ inline uint32_t quantise16(float p, float lo, float hi) {
+__device__ inline uint32_t quantise15(float p, float lo, float hi) {
     float u = (p - lo) / (hi - lo);
     u = fminf(fmaxf(u, 0.0f), 1.0f);  // NaN -> 0
-    return (uint32_t)(u * 65535.0f);
+    return (uint32_t)(u * 32767.0f);
 }
 
 __global__ void __launch_bounds__(256) morton_kernel(const float* __restrict__ s0, int64_t n,
                                                      int ax_a, int ax_b, float lo_a, float hi_a,
                                                      float lo_b, float hi_b,
-                                                     int64_t* __restrict__ keys) {
+                                                     int32_t* __restrict__ keys) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t qa = quantise16(s0[ax_a * n + i], lo_a, hi_a);
-    const uint32_t qb = quantise16(s0[ax_b * n + i], lo_b, hi_b);
-    keys[i] = (int64_t)(spread16(qa) | (spread16(qb) << 1));
+    const uint32_t qa = quantise15(s0[ax_a * n + i], lo_a, hi_a);
+    const uint32_t qb = quantise15(s0[ax_b * n + i], lo_b, hi_b);
+    keys[i] = (int32_t)(spread15(qa) | (spread15(qb) << 1));
 }
 
 }  // namespace slt
@@ -124,7 +124,7 @@ int64_t slt_grid_bytes(const slt_grid_desc* g) {
     return (int64_t)g->n[0] * g->n[1] * g->n[2] * 16;
 }
 
-int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int64_t* keys,
+int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int32_t* keys,
                     slt_stream_t stream) {
     clear_error();
     if (n < 0 || !g || (n > 0 && (!s0 || !keys))) {

@@ -218,10 +218,14 @@ def apply_ccd_mask(rf, weights=None, ccd_shape_m=(13.5e-3, 18e-3)):
 # ---------------------------------------------------------------------------
 
 def histogram2d_device(x: torch.Tensor, y: torch.Tensor, xedges: np.ndarray, yedges: np.ndarray,
-                       weights: torch.Tensor | None = None, mask: torch.Tensor | None = None):
+                       weights: torch.Tensor | None = None, mask: torch.Tensor | None = None,
+                       order: torch.Tensor | None = None):
     """numpy.histogram2d(x, y, bins=(xedges, yedges), weights).T on the GPU.
 
     x, y: float32 or float64 device vectors of one dtype (any stride).
+    order: optional int32 device permutation of the rays (the Morton order they
+    were traced in, ``TraceResult.order``): visiting rays in that order lets a
+    wavefront combine its atomics; counts are the same either way.
     Returns (nyb, nxb) tensors: counts as int64 and, when weights are given,
     fp64 weighted sums.  NaN positions are dropped, as values outside the edges.
     """
@@ -241,8 +245,13 @@ def histogram2d_device(x: torch.Tensor, y: torch.Tensor, xedges: np.ndarray, yed
     if mask is not None:
         m = mask.to(device=dev, dtype=torch.uint8).contiguous()
     n = x.numel()
+    o = None
+    if order is not None:
+        if order.numel() != n:
+            raise ValueError("order must hold one entry per ray")
+        o = order.to(device=dev, dtype=torch.int32).contiguous()
     N.check(N.lib().slt_histogram2d(x.data_ptr(), x.stride(0), y.data_ptr(), y.stride(0),
-                                    1 if x.dtype == torch.float64 else 0, n, N.ptr(m),
+                                    1 if x.dtype == torch.float64 else 0, n, N.ptr(o), N.ptr(m),
                                     xe.data_ptr(), nxb, ye.data_ptr(), nyb, N.ptr(w),
                                     counts.data_ptr(), N.ptr(wsum), N.stream_handle()))
     return counts.to(torch.int64) & 0xFFFFFFFF, wsum

@@ -234,7 +234,8 @@ class ServingEngine:
         inv = a.mat.grid.inv_brems
         rf, amp = jones_device(a.out_state, a.mat.grid.probing_direction,
                                a.out_tau if inv else None)
-        counts, wsum = histogram2d_device(rf[0], rf[2], a.mat.edges, a.mat.edges, weights=amp)
+        counts, wsum = histogram2d_device(rf[0], rf[2], a.mat.edges, a.mat.edges, weights=amp,
+                                          order=a.order)
         a.result = {"rf": rf if self.keep_outputs else None,
                     "amp": amp if self.keep_outputs else None,
                     "image": wsum if wsum is not None else counts,
@@ -370,7 +371,8 @@ def run_offline(requests, *, step_fraction: float = 0.5, device=None):
                            save_all=False, want_path=True)
         rf, amp = jones_device(res.states[0], mat.grid.probing_direction,
                                res.tau[0] if res.tau is not None else None)
-        counts, wsum = histogram2d_device(rf[0], rf[2], mat.edges, mat.edges, weights=amp)
+        counts, wsum = histogram2d_device(rf[0], rf[2], mat.edges, mat.edges, weights=amp,
+                                          order=res.order)
         out[req.rid] = {"rf": rf, "amp": amp, "image": wsum if wsum is not None else counts,
                         "path": res.path, "steps": res.total_steps, "state": res.states[0]}
     torch.cuda.synchronize(dev)

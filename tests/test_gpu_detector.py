@@ -114,3 +114,112 @@ def test_traced_detector_image_matches_oracle_except_edge_rays(orc):
         near |= np.min(np.abs(v[:, None] - e[None, :]), axis=1) <= tol
     diff = np.abs(H_gpu - H_ref.T).sum()
     assert diff <= 2 * near.sum(), (diff, near.sum())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ordered_histogram_same_counts_as_numpy(dtype):
+    """Visiting rays through an order (clustered, so wavefronts share cells, and random)
+    gives numpy's counts exactly and its weighted sums to summation order; a mask
+    still applies to the ray, not to the position in the order."""
+    import torch
+
+    from sloth_amd.processing.diagnostics import histogram2d_device
+
+    rng = np.random.default_rng(21)
+    n = 300_001
+    x = _edge_heavy_sample(n, -9.0, 9.0, 64, rng, dtype)
+    y = _edge_heavy_sample(n, -6.75, 6.75, 48, rng, dtype)
+    xe, ye = np.linspace(-9.0, 9.0, 65), np.linspace(-6.75, 6.75, 49)
+    w = rng.uniform(0, 1, n)
+    mask = rng.uniform(size=n) < 0.9
+    keep = ~np.isnan(x) & ~np.isnan(y) & mask
+    H, _, _ = np.histogram2d(x[keep], y[keep], bins=[xe, ye])
+    Hw, _, _ = np.histogram2d(x[keep], y[keep], bins=[xe, ye], weights=w[keep])
+    clustered = np.lexsort((np.nan_to_num(y), np.nan_to_num(x)))  # runs of equal cells
+    for order in (clustered, rng.permutation(n)):
+        counts, wsum = histogram2d_device(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(),
+                                          xe, ye, weights=torch.from_numpy(w),
+                                          mask=torch.from_numpy(mask),
+                                          order=torch.from_numpy(order.astype(np.int32)))
+        np.testing.assert_array_equal(counts.cpu().numpy(), H.T.astype(np.int64))
+        np.testing.assert_allclose(wsum.cpu().numpy(), Hw.T, rtol=1e-12, atol=1e-12)
+        counts_only, none = histogram2d_device(torch.from_numpy(x).cuda(),
+                                               torch.from_numpy(y).cuda(), xe, ye,
+                                               mask=torch.from_numpy(mask),
+                                               order=torch.from_numpy(order.astype(np.int32)))
+        assert none is None
+        np.testing.assert_array_equal(counts_only.cpu().numpy(), H.T.astype(np.int64))
+
+
+def test_ordered_histogram_skips_entries_outside_the_rays():
+    """Entries of order outside [0, n) are skipped; a repeated entry counts twice."""
+    import torch
+
+    from sloth_amd.processing.diagnostics import histogram2d_device
+
+    rng = np.random.default_rng(22)
+    n = 1000
+    x = rng.uniform(-1, 1, n).astype(np.float32)
+    order = np.arange(n, dtype=np.int32)
+    order[10], order[20], order[30] = -1, n, 5
+    e = np.linspace(-1, 1, 9)
+    counts, _ = histogram2d_device(torch.from_numpy(x).cuda(), torch.from_numpy(x).cuda(), e, e,
+                                   order=torch.from_numpy(order))
+    vis = order[(order >= 0) & (order < n)]
+    H, _, _ = np.histogram2d(x[vis], x[vis], bins=[e, e])
+    np.testing.assert_array_equal(counts.cpu().numpy(), H.T.astype(np.int64))
+    with pytest.raises(ValueError):
+        histogram2d_device(torch.from_numpy(x).cuda(), torch.from_numpy(x).cuda(), e, e,
+                           order=torch.from_numpy(order[:-1]))
+
+
+def _morton_keys_np(p, q, lo_a, hi_a, lo_b, hi_b):
+    """15-bit quantisation in float32 and bit interleaving (a on even bits)."""
+    def quant(v, lo, hi):
+        with np.errstate(invalid="ignore"):
+            u = (v - np.float32(lo)) / (np.float32(hi) - np.float32(lo))
+        u = np.where(np.isnan(u), np.float32(0), np.clip(u, np.float32(0), np.float32(1)))
+        return (u.astype(np.float32) * np.float32(32767.0)).astype(np.uint32)
+
+    qa, qb = quant(p, lo_a, hi_a), quant(q, lo_b, hi_b)
+    key = np.zeros(p.shape, dtype=np.int64)
+    for bit in range(15):
+        key |= ((qa.astype(np.int64) >> bit) & 1) << (2 * bit)
+        key |= ((qb.astype(np.int64) >> bit) & 1) << (2 * bit + 1)
+    return key
+
+
+def test_morton_keys_and_ray_order():
+    """Keys are the 30-bit interleave of the 15-bit transverse positions (clamped, NaN
+    to 0); ray_order is the stable sort of them."""
+    import torch
+
+    from sloth_amd import _native as N
+    from sloth_amd.core import engine
+    from sloth_amd.fields import sine_field
+
+    L = 4e-3
+    ne, coords = sine_field(16, L)
+    grid = engine.PackedGrid(ne, *coords, 2 * np.pi * C / 1064e-9)
+    rng = np.random.default_rng(23)
+    n = 50_000
+    s0 = np.zeros((6, n), dtype=np.float32)
+    s0[0] = rng.uniform(-0.6 * L, 0.6 * L, n)     # some clamp at either side
+    s0[1] = rng.uniform(-0.6 * L, 0.6 * L, n)
+    s0[1, :7] = np.nan
+    s0[0, 7:20] = s0[0, 20:33]                     # equal keys keep their input order
+    s0[1, 7:20] = s0[1, 20:33]
+    s0[2] = -L / 2
+    s0[5] = C
+    t = torch.from_numpy(s0).cuda()
+    keys = torch.empty(n, dtype=torch.int32, device="cuda")
+    import ctypes
+    N.check(N.lib().slt_morton_keys(t.data_ptr(), n, ctypes.byref(grid.desc), keys.data_ptr(),
+                                    None))
+    d = grid.desc
+    ref = _morton_keys_np(s0[0], s0[1], d.lo[0], d.hi[0], d.lo[1], d.hi[1])
+    got = keys.cpu().numpy().astype(np.int64)
+    np.testing.assert_array_equal(got, ref)
+    assert got.min() >= 0 and got.max() < (1 << 30)
+    order = engine.ray_order(grid, t).cpu().numpy()
+    np.testing.assert_array_equal(order, np.argsort(ref, kind="stable"))
