@@ -98,7 +98,8 @@ struct AdjArgs {
 };
 
 template <bool KAPPA>
-__global__ void __launch_bounds__(256) forward_ckpt_kernel(AdjArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+forward_ckpt_kernel(AdjArgs A) {
     const int lb = xcd_block(blockIdx.x, gridDim.x, A.nblocks);
     if (lb < 0) return;
     const int64_t t = (int64_t)lb * blockDim.x + threadIdx.x;

@@ -114,31 +114,35 @@ __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, flo
     float tx = u0 - cc.c0;
     float ty = u1 - cc.c1;
     float tz = u2 - cc.c2;
+    bool inside = true;
     if (!(unit_interval(tx) & unit_interval(ty) & unit_interval(tz))) {
         if (CHECK && !in_box(G, x, y, z)) {
-            out = make_float4(0.f, 0.f, 0.f, 0.f);
-            return false;
+            inside = false;
+        } else {
+            // inside the box u > -1 (a point on the lower face may round a hair below 0):
+            // truncation is floor for u >= 0 and gives cell 0 for the rest
+            const int i0 = min((int)u0, G.n0 - 2);
+            const int i1 = min((int)u1, G.n1 - 2);
+            const int i2 = min((int)u2, G.n2 - 2);
+            cc.c0 = (float)i0;
+            cc.c1 = (float)i1;
+            cc.c2 = (float)i2;
+            cc.face = (min(min(i0, i1), i2) == 0) | (i0 == G.n0 - 2) | (i1 == G.n1 - 2) |
+                      (i2 == G.n2 - 2);
+            tx = u0 - cc.c0;
+            ty = u1 - cc.c1;
+            tz = u2 - cc.c2;
+            load_cell(G, (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2)),
+                      cc);
         }
-        // inside the box u > -1 (a point on the lower face may round a hair below 0):
-        // truncation is floor for u >= 0 and gives cell 0 for the rest
-        const int i0 = min((int)u0, G.n0 - 2);
-        const int i1 = min((int)u1, G.n1 - 2);
-        const int i2 = min((int)u2, G.n2 - 2);
-        cc.c0 = (float)i0;
-        cc.c1 = (float)i1;
-        cc.c2 = (float)i2;
-        cc.face = (min(min(i0, i1), i2) == 0) | (i0 == G.n0 - 2) | (i1 == G.n1 - 2) |
-                  (i2 == G.n2 - 2);
-        tx = u0 - cc.c0;
-        ty = u1 - cc.c1;
-        tz = u2 - cc.c2;
-        load_cell(G, (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2)), cc);
     } else if (CHECK && cc.face && !in_box(G, x, y, z)) {
-        out = make_float4(0.f, 0.f, 0.f, 0.f);
-        return false;
+        inside = false;
     }
+    // evaluated for every point; a point outside the box then gets zero in a branch
+    // that is skipped unless a lane needs it
     poly_eval<KAPPA>(cc, tx, ty, tz, out);
-    return true;
+    if (CHECK && __builtin_expect(!inside, 0)) out = make_float4(0.f, 0.f, 0.f, 0.f);
+    return inside;
 }
 
 // The same with the adjoint's cell cache (keyed by voxel index).
