@@ -85,9 +85,12 @@ int64_t slt_grid_bytes(const slt_grid_desc* g);
  *   omega, e_charge, epsilon_0, speed_of_light   physical constants (fp64)
  *   coulomb_log   > 0 overrides the Coulomb logarithm (extension); <= 0 uses
  *                 the reference's max(2, ln(v_the / (omega_max b_min)))
- *   packed        device voxel buffer (slt_grid_bytes(g) bytes) or NULL
+ *   packed        device voxel buffer (slt_grid_bytes(g) bytes) or NULL; voxel
+ *                 = (d/dx * dx, d/dy * dy, d/dz * dz, kappa) with d_a the cell
+ *                 size (hi - lo) / (n - 1) of g: the gradient in grid units, the
+ *                 form the integrators step in
  *   planar[4]     optional device fp32 (nx,ny,nz) C-order outputs for
- *                 d/dx, d/dy, d/dz, kappa (each may be NULL)
+ *                 d/dx, d/dy, d/dz (physical units) and kappa (each may be NULL)
  * Arithmetic is fp64; results are rounded once to fp32.
  */
 int slt_grid_pack(const float* ne, const float* cx, const float* cy, const float* cz,
@@ -248,10 +251,11 @@ int64_t slt_ray_state_bytes(int64_t n_rays);
 
 /*
  * Initialise resumable state: row t takes ray order[t] of s0 (6 x n, row stride
- * n), or ray t if order is NULL.
+ * n), or ray t if order is NULL.  The state is kept in the grid units of g (the
+ * descriptor of the job's packed grid), so the same g goes into the job.
  */
 int slt_ray_state_init(void* state, int64_t n_rays, const float* s0, const int32_t* order,
-                       slt_stream_t stream);
+                       const slt_grid_desc* g, slt_stream_t stream);
 
 /* One request's rays and the step range a step batch advances them through. */
 typedef struct slt_job {

@@ -165,7 +165,7 @@ def lib():
     L.slt_beam.argtypes = [i64, i32, i32, dbl, dbl, dbl, dbl, ctypes.c_uint64, vp, vp]
     L.slt_ray_state_bytes.argtypes = [i64]
     L.slt_ray_state_bytes.restype = i64
-    L.slt_ray_state_init.argtypes = [vp, i64, vp, vp, vp]
+    L.slt_ray_state_init.argtypes = [vp, i64, vp, vp, ctypes.POINTER(GridDesc), vp]
     L.slt_job_table_prepare.argtypes = [ctypes.POINTER(JobTable)]
     L.slt_advance.argtypes = [vp, i32, vp]
     L.slt_checkpoint_bytes.argtypes = [i64, i64, i32]

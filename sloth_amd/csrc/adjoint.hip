@@ -107,8 +107,8 @@ forward_ckpt_kernel(AdjArgs A) {
     if (t < A.n) {
         const int64_t ray = A.order ? (int64_t)A.order[t] : t;
         RayState s;
-        state_from_rays(A.s0, A.n, ray, s);
-        SaveSink sink{A.final_state, A.final_tau, A.n, ray, 0, A.nseg};
+        state_from_rays(A.G, A.s0, A.n, ray, s);
+        SaveSink sink{A.final_state, A.final_tau, A.n, ray, 0, A.nseg, A.s0 + 3 * A.n, A.n, ray};
         for (int c = 0; c < A.nck; ++c) {
             const CkptView v = ckpt_view(A.ckpt, A.n, c);
 #pragma unroll
@@ -309,20 +309,17 @@ __device__ __forceinline__ void accumulate(const DevGrid& G, const AdjOut& out, 
     }
 }
 
-// Adjoint of one RHS evaluation at p with output adjoint Ab (gx, gy, gz, kappa):
-// returns d(loss)/d(p) in rb and accumulates the grid adjoint.  Every lane of the
-// wavefront calls it together; `act` lanes contribute.
-__device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float x, float y, float z,
+// Adjoint of one RHS evaluation at the grid-unit point u with output adjoint Ab
+// (gx, gy, gz, kappa): returns d(loss)/d(u) in rb and accumulates the grid
+// adjoint.  Every lane of the wavefront calls it together; `act` lanes contribute.
+__device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float u0, float u1, float u2,
                                          const float4& Ab, Cell& cc, Moments& M,
                                          const AdjOut& out, float rb[3]) {
     rb[0] = rb[1] = rb[2] = 0.f;
-    const bool in = act && in_box(G, x, y, z);
+    const bool in = act && in_box(G, u0, u1, u2);
     int id = -1;
     float tx = 0.f, ty = 0.f, tz = 0.f;
     if (in) {
-        const float u0 = fmaf(x, G.id0, G.ul0);
-        const float u1 = fmaf(y, G.id1, G.ul1);
-        const float u2 = fmaf(z, G.id2, G.ul2);
         const int i0 = min((int)u0, G.n0 - 2);
         const int i1 = min((int)u1, G.n1 - 2);
         const int i2 = min((int)u2, G.n2 - 2);
@@ -340,10 +337,10 @@ __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float x, fl
         };
         const float Kx = dot(cc.kx), Ky = dot(cc.ky), Kz = dot(cc.kz);
         const float Kxy = dot(cc.kxy), Kxz = dot(cc.kxz), Kyz = dot(cc.kyz), Kxyz = dot(cc.kxyz);
-        // d f / d t_a of the cell polynomial
-        rb[0] = G.id0 * fmaf(tz, fmaf(ty, Kxyz, Kxz), fmaf(ty, Kxy, Kx));
-        rb[1] = G.id1 * fmaf(tz, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxy, Ky));
-        rb[2] = G.id2 * fmaf(ty, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxz, Kz));
+        // d f / d t_a of the cell polynomial (t = u - corner, so d/du = d/dt)
+        rb[0] = fmaf(tz, fmaf(ty, Kxyz, Kxz), fmaf(ty, Kxy, Kx));
+        rb[1] = fmaf(tz, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxy, Ky));
+        rb[2] = fmaf(ty, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxz, Kz));
     }
     accumulate(G, out, M, in, id, tx, ty, tz, Ab);
 }
@@ -424,11 +421,13 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
     float rb[3] = {0.f, 0.f, 0.f}, vb[3] = {0.f, 0.f, 0.f}, v0[3] = {0.f, 0.f, 0.f};
     float tb = 0.f;
     if (valid) {
+        // seeds are d(loss)/d(final x, v) in metres; the state is in grid units
+        // (x = lo + u d, v = v_u d), so d/du = d d/dx and d/dv_u = d d/dv
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            rb[c] = A.seed_state[c * n + ray];
-            vb[c] = A.seed_state[(3 + c) * n + ray];
-            v0[c] = A.s0[(3 + c) * n + ray];
+            rb[c] = (float)((double)A.seed_state[c * n + ray] * A.G.dsp[c]);
+            vb[c] = (float)((double)A.seed_state[(3 + c) * n + ray] * A.G.dsp[c]);
+            v0[c] = velocity_units(A.G, c, A.s0[(3 + c) * n + ray]);
         }
         tb = A.seed_tau ? (float)A.seed_tau[ray] : 0.f;
     }
@@ -436,7 +435,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
     cc.id = -1;
     Moments M;
     M.id = -1;
-    const SaveSink none{nullptr, nullptr, n, ray, 0, A.nseg};
+    const SaveSink none{nullptr, nullptr, n, ray, 0, A.nseg, A.s0 + 3 * n, n, ray};
     for (int c = A.nck - 1; c >= 0; --c) {
         GlobalTape tape{A.tape, n, tt, 0};
         if (valid) {
@@ -499,6 +498,7 @@ struct UnpackArgs {
     double omega, e_charge, epsilon_0, c_light, coulomb_log;
     int n0, n1, n2;
     int64_t s0, s1, s2;
+    double isp[3];  // the pack's per-axis scale of the gradient channels (1 / cell size)
     float* __restrict__ dne;
 };
 
@@ -557,7 +557,8 @@ __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs A) {
             for (int i = j - 1; i <= j + 1; ++i) {
                 if (i < 0 || i >= nn[a]) continue;
                 const double w = grad_coef(cs[a], nn[a], i, j);
-                if (w != 0.0) f += w * (double)A.adj[(pk + (int64_t)(i - j) * ps[a]) * 4 + a];
+                if (w != 0.0)
+                    f += w * A.isp[a] * (double)A.adj[(pk + (int64_t)(i - j) * ps[a]) * 4 + a];
             }
         }
         double d = A.coeff * f;
@@ -690,6 +691,13 @@ extern "C" int slt_adjoint_unpack(const float* grid_adj, const slt_grid_desc* g,
     A.coulomb_log = coulomb_log;
     A.n0 = g->n[0]; A.n1 = g->n[1]; A.n2 = g->n[2];
     A.s0 = g->stride[0]; A.s1 = g->stride[1]; A.s2 = g->stride[2];
+    for (int ax = 0; ax < 3; ++ax) {
+        if (g->n[ax] < 2 || !(g->hi[ax] > g->lo[ax])) {
+            set_error("slt_adjoint_unpack: grid axis %d needs >= 2 points and hi > lo", ax);
+            return SLT_ERR_INVALID;
+        }
+        A.isp[ax] = grid_inv_spacing(g, ax);
+    }
     A.dne = dne;
     int64_t blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;

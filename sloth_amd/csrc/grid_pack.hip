@@ -21,6 +21,7 @@ struct PackArgs {
     double omega, e_charge, epsilon_0, c_light, coulomb_log;
     int n0, n1, n2;
     int64_t s0, s1, s2;  // packed strides
+    double isp[3];       // per-axis scale of the packed gradient channels (1 / cell size)
     float4* __restrict__ packed;
     float* __restrict__ pdx;
     float* __restrict__ pdy;
@@ -85,12 +86,16 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs A) {
             const double z = (double)A.zz[A.n_z == 1 ? 0 : lin];
             kap = kappa_ib((double)A.ne[lin], te, z, A);
         }
-        const float4 v = make_float4((float)gx, (float)gy, (float)gz, (float)kap);
-        if (A.packed) A.packed[i * A.s0 + j * A.s1 + k * A.s2] = v;
-        if (A.pdx) A.pdx[lin] = v.x;
-        if (A.pdy) A.pdy[lin] = v.y;
-        if (A.pdz) A.pdz[lin] = v.z;
-        if (A.pk) A.pk[lin] = v.w;
+        // packed: the gradient in grid units (per cell, as the integrators step);
+        // planar: the physical gradient
+        if (A.packed)
+            A.packed[i * A.s0 + j * A.s1 + k * A.s2] =
+                make_float4((float)(gx * A.isp[0]), (float)(gy * A.isp[1]),
+                            (float)(gz * A.isp[2]), (float)kap);
+        if (A.pdx) A.pdx[lin] = (float)gx;
+        if (A.pdy) A.pdy[lin] = (float)gy;
+        if (A.pdz) A.pdz[lin] = (float)gz;
+        if (A.pk) A.pk[lin] = (float)kap;
     }
 }
 
@@ -172,6 +177,13 @@ extern "C" int slt_grid_pack(const float* ne, const float* cx, const float* cy, 
     A.coulomb_log = coulomb_log;
     A.n0 = g->n[0]; A.n1 = g->n[1]; A.n2 = g->n[2];
     A.s0 = g->stride[0]; A.s1 = g->stride[1]; A.s2 = g->stride[2];
+    for (int a = 0; a < 3; ++a) {
+        if (!(g->hi[a] > g->lo[a])) {
+            set_error("slt_grid_pack: grid axis %d needs hi > lo", a);
+            return SLT_ERR_INVALID;
+        }
+        A.isp[a] = grid_inv_spacing(g, a);
+    }
     A.packed = static_cast<float4*>(packed);
     A.pdx = planar_dx; A.pdy = planar_dy; A.pdz = planar_dz; A.pk = planar_kappa;
     int64_t blocks = (total + 255) / 256;

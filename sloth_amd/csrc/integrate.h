@@ -74,9 +74,10 @@ __device__ __forceinline__ float tri(float tx, float ty, float tz, float k0, flo
     return fmaf(tz, b, fmaf(ty, c, fmaf(tx, kx, k0)));
 }
 
-__device__ __forceinline__ bool in_box(const DevGrid& G, float x, float y, float z) {
-    return (x >= G.lo0) & (x <= G.hi0) & (y >= G.lo1) & (y <= G.hi1) & (z >= G.lo2) &
-           (z <= G.hi2);
+// The closed box in grid units: 0 <= u <= n - 1 on every axis.
+__device__ __forceinline__ bool in_box(const DevGrid& G, float u0, float u1, float u2) {
+    return (u0 >= 0.f) & (u0 <= G.ub0) & (u1 >= 0.f) & (u1 <= G.ub1) & (u2 >= 0.f) &
+           (u2 <= G.ub2);
 }
 
 // (gx, gy, gz, kappa) of the cell polynomial at local coordinates t.
@@ -103,24 +104,21 @@ __device__ __forceinline__ unsigned unit_interval(float t) {
     return __float_as_uint(t) < 0x3f800000u ? 1u : 0u;
 }
 
-// Interpolated (gx, gy, gz, kappa) at (x, y, z); zero and false outside the
-// closed box.  CHECK = false when the caller already knows the point is inside.
+// Interpolated (gx, gy, gz, kappa) at the grid-unit point (u0, u1, u2); zero and
+// false outside the closed box.  CHECK = false when the caller already knows the
+// point is inside.
 template <bool KAPPA, bool CHECK>
-__device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, float z, FwdCell& cc,
-                                         float4& out) {
-    const float u0 = fmaf(x, G.id0, G.ul0);
-    const float u1 = fmaf(y, G.id1, G.ul1);
-    const float u2 = fmaf(z, G.id2, G.ul2);
+__device__ __forceinline__ bool eval_rhs(const DevGrid& G, float u0, float u1, float u2,
+                                         FwdCell& cc, float4& out) {
     float tx = u0 - cc.c0;
     float ty = u1 - cc.c1;
     float tz = u2 - cc.c2;
     bool inside = true;
     if (!(unit_interval(tx) & unit_interval(ty) & unit_interval(tz))) {
-        if (CHECK && !in_box(G, x, y, z)) {
+        if (CHECK && !in_box(G, u0, u1, u2)) {
             inside = false;
         } else {
-            // inside the box u > -1 (a point on the lower face may round a hair below 0):
-            // truncation is floor for u >= 0 and gives cell 0 for the rest
+            // inside the box u >= 0, so truncation is floor
             const int i0 = min((int)u0, G.n0 - 2);
             const int i1 = min((int)u1, G.n1 - 2);
             const int i2 = min((int)u2, G.n2 - 2);
@@ -135,7 +133,7 @@ __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, flo
             load_cell(G, (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2)),
                       cc);
         }
-    } else if (CHECK && cc.face && !in_box(G, x, y, z)) {
+    } else if (CHECK && cc.face && !in_box(G, u0, u1, u2)) {
         inside = false;
     }
     // evaluated for every point; a point outside the box then gets zero in a branch
@@ -147,15 +145,12 @@ __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, flo
 
 // The same with the adjoint's cell cache (keyed by voxel index).
 template <bool KAPPA, bool CHECK>
-__device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, float z, Cell& cc,
+__device__ __forceinline__ bool eval_rhs(const DevGrid& G, float u0, float u1, float u2, Cell& cc,
                                          float4& out) {
-    if (CHECK && !in_box(G, x, y, z)) {
+    if (CHECK && !in_box(G, u0, u1, u2)) {
         out = make_float4(0.f, 0.f, 0.f, 0.f);
         return false;
     }
-    const float u0 = fmaf(x, G.id0, G.ul0);
-    const float u1 = fmaf(y, G.id1, G.ul1);
-    const float u2 = fmaf(z, G.id2, G.ul2);
     const int i0 = min((int)u0, G.n0 - 2);
     const int i1 = min((int)u1, G.n1 - 2);
     const int i2 = min((int)u2, G.n2 - 2);
@@ -173,8 +168,8 @@ __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float x, float y, flo
 // Conservative: under-estimating only means an exact RK4 step with zero RHS.
 __device__ __forceinline__ int ballistic_steps(const DevGrid& G, const float r[3],
                                                const float v[3], float h, int kmax) {
-    const float lo[3] = {G.lo0, G.lo1, G.lo2};
-    const float hi[3] = {G.hi0, G.hi1, G.hi2};
+    const float lo[3] = {0.f, 0.f, 0.f};
+    const float hi[3] = {G.ub0, G.ub1, G.ub2};
     float tmin = 0.f, tmax = INFINITY;
     bool never = false;
 #pragma unroll
@@ -198,9 +193,9 @@ __device__ __forceinline__ int ballistic_steps(const DevGrid& G, const float r[3
 // Per-ray integration state.  In the offline kernel it lives in VGPRs for the
 // whole trajectory; the serving kernel loads and stores it around each step batch.
 struct RayState {
-    double rb[3];     // position base [m]
+    double rb[3];     // position base [grid units]
     float off[3];     // position offset since the last fold into rb
-    float v0[3];      // start velocity [m/s]
+    float v0[3];      // start velocity [grid units/s]
     float dv[3];      // velocity change since the start
     double tau;       // optical depth
     double path;      // path length [m]
@@ -208,13 +203,18 @@ struct RayState {
     uint32_t fold;    // RK4 steps since the start (fold phase of off into rb)
 };
 
-__device__ __forceinline__ void state_from_rays(const float* __restrict__ s0, int64_t n,
-                                                int64_t ray, RayState& s) {
+// Start velocity component c in grid units/s (fp32), as the state holds it.
+__device__ __forceinline__ float velocity_units(const DevGrid& G, int c, float v) {
+    return (float)((double)v * G.isp[c]);
+}
+
+__device__ __forceinline__ void state_from_rays(const DevGrid& G, const float* __restrict__ s0,
+                                                int64_t n, int64_t ray, RayState& s) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        s.rb[c] = (double)s0[c * n + ray];
+        s.rb[c] = ((double)s0[c * n + ray] - G.org[c]) * G.isp[c];
         s.off[c] = 0.f;
-        s.v0[c] = s0[(3 + c) * n + ray];
+        s.v0[c] = velocity_units(G, c, s0[(3 + c) * n + ray]);
         s.dv[c] = 0.f;
     }
     s.tau = 0.0;
@@ -223,7 +223,10 @@ __device__ __forceinline__ void state_from_rays(const float* __restrict__ s0, in
     s.fold = 0;
 }
 
-// Writes the ray state at save points: save_state is (S, 6, n), save_tau (S, n).
+// Writes the ray state at save points, in metres and m/s: save_state is
+// (S, 6, n), save_tau (S, n).  The velocity written is the caller's start
+// velocity (v0m[c * v0_ld + v0_i], m/s) plus the change, so a ray that meets no
+// gradient keeps its input velocity bit for bit.
 struct SaveSink {
     float* __restrict__ state;
     double* __restrict__ tau;
@@ -231,15 +234,19 @@ struct SaveSink {
     int64_t ray;
     int every;  // 1: slot = save index; 0: only the last save, in slot 0
     int last;   // index of the last save point (= n_seg)
+    const float* __restrict__ v0m;
+    int64_t v0_ld;
+    int64_t v0_i;
 
-    __device__ __forceinline__ void put(int idx, const RayState& s) const {
+    __device__ __forceinline__ void put(int idx, const RayState& s, const DevGrid& G) const {
         if (!state || (!every && idx != last)) return;
         const int64_t slot = every ? idx : 0;
         float* base = state + slot * 6 * n;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            base[c * n + ray] = (float)(s.rb[c] + (double)s.off[c]);
-            base[(3 + c) * n + ray] = s.v0[c] + s.dv[c];
+            base[c * n + ray] = (float)(G.org[c] + (s.rb[c] + (double)s.off[c]) * G.dsp[c]);
+            base[(3 + c) * n + ray] =
+                (float)((double)v0m[c * v0_ld + v0_i] + (double)s.dv[c] * G.dsp[c]);
         }
         if (tau) tau[slot * n + ray] = s.tau;
     }
@@ -300,7 +307,8 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                             const double vc = (double)s.v0[c] + (double)s.dv[c];
                             s.rb[c] += T * vc;
                             rbf[c] = (float)s.rb[c];
-                            sp2 += vc * vc;
+                            const double vm = vc * G.dsp[c];  // m/s
+                            sp2 += vm * vm;
                         }
                         if (PATH) s.path += T * sqrt(sp2);
                         tape.straight((float)T);
@@ -329,8 +337,11 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                 s.off[0] += dx;
                 s.off[1] += dy;
                 s.off[2] += dz;
-                // path length: the polyline through the step end points
-                if (PATH) s.path += (double)__builtin_amdgcn_sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
+                // path length: the polyline through the step end points (in metres)
+                if (PATH) {
+                    const float ex = dx * G.sp0, ey = dy * G.sp1, ez = dz * G.sp2;
+                    s.path += (double)__builtin_amdgcn_sqrtf(fmaf(ex, ex, fmaf(ey, ey, ez * ez)));
+                }
                 s.dv[0] += h6 * (A1.x + 2.f * (A2.x + A3.x) + A4.x);
                 s.dv[1] += h6 * (A1.y + 2.f * (A2.y + A3.y) + A4.y);
                 s.dv[2] += h6 * (A1.z + 2.f * (A2.z + A3.z) + A4.z);
@@ -346,7 +357,7 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                     }
                 }
             }
-            if (seg_hi <= g_end) sink.put(seg + 1, s);
+            if (seg_hi <= g_end) sink.put(seg + 1, s, G);
         }
         seg_lo = seg_hi;
     }

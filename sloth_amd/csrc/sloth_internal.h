@@ -18,16 +18,43 @@ int check_launch(const char* what);
 // Device view of a packed grid.  Indices fit in 32 bits and every stride and
 // extent in 24 (so voxel offsets use 24-bit multiplies): the host rejects
 // larger grids.
+//
+// The integrators work in grid units: a position is u = (x - lo) / d per axis
+// (d = (hi - lo) / (n - 1), the cell size), so node i sits at u = i and the
+// box is [0, n - 1]; velocities are v / d and the packed gradient channels
+// hold the gradient times 1 / d (slt_grid_pack).  A stage point is then its own
+// grid coordinate and needs no scaling before the cell lookup.  Conversions to
+// and from metres happen once per ray (start state, save points) in fp64.
 struct DevGrid {
     const float4* __restrict__ vox;
     int n0, n1, n2;
     int s0, s1, s2;
-    float lo0, lo1, lo2;
-    float hi0, hi1, hi2;
-    float id0, id1, id2;
-    float ul0, ul1, ul2;  // -lo * inv_d: the grid coordinate is u = fma(x, inv_d, ul)
+    float ub0, ub1, ub2;  // upper box face in grid units: n - 1 (the lower one is 0)
+    float sp0, sp1, sp2;  // cell size [m] (path length in fp32)
+    double org[3];        // grid origin lo [m]
+    double dsp[3];        // cell size d [m]
+    double isp[3];        // 1 / d [1/m]
     int nchan;
 };
+
+// Unit conversion fields of a DevGrid from a descriptor (host or device).
+__host__ __device__ inline void fill_grid_units(const slt_grid_desc* g, DevGrid* d) {
+    const int n[3] = {g->n[0], g->n[1], g->n[2]};
+    for (int a = 0; a < 3; ++a) {
+        const double lo = (double)g->lo[a], hi = (double)g->hi[a];
+        const double m = (double)(n[a] - 1);
+        d->org[a] = lo;
+        d->dsp[a] = (hi - lo) / m;
+        d->isp[a] = m / (hi - lo);
+    }
+    d->ub0 = (float)(n[0] - 1); d->ub1 = (float)(n[1] - 1); d->ub2 = (float)(n[2] - 1);
+    d->sp0 = (float)d->dsp[0]; d->sp1 = (float)d->dsp[1]; d->sp2 = (float)d->dsp[2];
+}
+
+// 1 / cell size of axis a, as fill_grid_units computes it (the pack's scale).
+__host__ __device__ inline double grid_inv_spacing(const slt_grid_desc* g, int a) {
+    return (double)(g->n[a] - 1) / ((double)g->hi[a] - (double)g->lo[a]);
+}
 
 inline int make_dev_grid(const void* packed, const slt_grid_desc* g, DevGrid* d) {
     if (!packed || !g) {
@@ -45,14 +72,15 @@ inline int make_dev_grid(const void* packed, const slt_grid_desc* g, DevGrid* d)
                       (long long)g->stride[a], g->n[a]);
             return SLT_ERR_UNSUPPORTED;
         }
+        if (g->n[a] < 2 || !(g->hi[a] > g->lo[a])) {
+            set_error("grid axis %d: need >= 2 points and hi > lo", a);
+            return SLT_ERR_INVALID;
+        }
     }
     d->vox = static_cast<const float4*>(packed);
     d->n0 = g->n[0]; d->n1 = g->n[1]; d->n2 = g->n[2];
     d->s0 = (int)g->stride[0]; d->s1 = (int)g->stride[1]; d->s2 = (int)g->stride[2];
-    d->lo0 = g->lo[0]; d->lo1 = g->lo[1]; d->lo2 = g->lo[2];
-    d->hi0 = g->hi[0]; d->hi1 = g->hi[1]; d->hi2 = g->hi[2];
-    d->id0 = g->inv_d[0]; d->id1 = g->inv_d[1]; d->id2 = g->inv_d[2];
-    d->ul0 = -(d->lo0 * d->id0); d->ul1 = -(d->lo1 * d->id1); d->ul2 = -(d->lo2 * d->id2);
+    fill_grid_units(g, d);
     d->nchan = g->nchan;
     return SLT_OK;
 }
@@ -64,10 +92,7 @@ __host__ __device__ inline void make_dev_grid_device(const void* packed, const s
     d->vox = static_cast<const float4*>(packed);
     d->n0 = g->n[0]; d->n1 = g->n[1]; d->n2 = g->n[2];
     d->s0 = (int)g->stride[0]; d->s1 = (int)g->stride[1]; d->s2 = (int)g->stride[2];
-    d->lo0 = g->lo[0]; d->lo1 = g->lo[1]; d->lo2 = g->lo[2];
-    d->hi0 = g->hi[0]; d->hi1 = g->hi[1]; d->hi2 = g->hi[2];
-    d->id0 = g->inv_d[0]; d->id1 = g->inv_d[1]; d->id2 = g->inv_d[2];
-    d->ul0 = -(d->lo0 * d->id0); d->ul1 = -(d->lo1 * d->id1); d->ul2 = -(d->lo2 * d->id2);
+    fill_grid_units(g, d);
     d->nchan = g->nchan;
 }
 

@@ -16,7 +16,10 @@
 //    axis of the packed layout), so the gathers per RK4 step are ~1/8 of the
 //    4 x 8 corner loads the algorithm names, and an evaluation is 7 FMAs per
 //    channel.
-//  * Position = fp64 base + fp32 offset: the offset takes each step's
+//  * The state is in grid units (position (x - lo) / d, velocity v / d, with
+//    the packed gradient pre-scaled by 1 / d), so every stage point is its own
+//    grid coordinate: the cell test is a subtraction, with no per-stage scaling.
+//    Position = fp64 base + fp32 offset: the offset takes each step's
 //    increment and is folded into the base every 8 steps, so rounding stays at
 //    the offset's (small) magnitude.  Velocity = fixed start velocity + fp32
 //    change.  tau and path length accumulate in fp64.  All RHS arithmetic is
@@ -66,9 +69,10 @@ trace_kernel(TraceK K) {
         const int64_t n = K.n;
         const int64_t ray = K.order ? (int64_t)K.order[t] : t;
         RayState s;
-        state_from_rays(K.s0, n, ray, s);
-        SaveSink sink{K.save_state, K.save_tau, n, ray, K.save_every, K.nseg};
-        sink.put(0, s);
+        state_from_rays(K.G, K.s0, n, ray, s);
+        SaveSink sink{K.save_state, K.save_tau, n, ray, K.save_every, K.nseg, K.s0 + 3 * n, n,
+                      ray};
+        sink.put(0, s, K.G);
         int64_t total = 0;
         for (int q = 0; q < K.nseg; ++q) total += K.seg_nsub[q];
         integrate<KAPPA, PATH>(K.G, s, K.seg_nsub, K.seg_h, K.nseg, 0, total, sink);
@@ -82,6 +86,8 @@ trace_kernel(TraceK K) {
 // ---------------------------------------------------------------------------
 // Resumable ray state (serving): structure-of-arrays in one buffer.
 //   rb[3] f64 | tau f64 | path f64 | off[3] f32 | v0[3] f32 | dv[3] f32 | steps u32 | fold u32
+// rb, off and dv are in the grid units of the job's grid; v0 is the start
+// velocity in m/s (converted on load, and the base of the saved velocity).
 // ---------------------------------------------------------------------------
 struct StateView {
     SLT_GLOBAL double* rb[3];
@@ -114,12 +120,13 @@ __device__ __forceinline__ StateView state_view(void* base, int64_t n) {
     return v;
 }
 
-__device__ __forceinline__ void state_load(const StateView& v, int64_t i, RayState& s) {
+__device__ __forceinline__ void state_load(const StateView& v, int64_t i, const DevGrid& G,
+                                           RayState& s) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         s.rb[c] = v.rb[c][i];
         s.off[c] = v.off[c][i];
-        s.v0[c] = v.v0[c][i];
+        s.v0[c] = velocity_units(G, c, v.v0[c][i]);
         s.dv[c] = v.dv[c][i];
     }
     s.tau = v.tau[i];
@@ -141,18 +148,18 @@ __device__ __forceinline__ void state_store(const StateView& v, int64_t i, const
     v.fold[i] = s.fold;
 }
 
-__global__ void __launch_bounds__(256) state_init_kernel(void* base, int64_t n,
+__global__ void __launch_bounds__(256) state_init_kernel(DevGrid G, void* base, int64_t n,
                                                          const float* __restrict__ s0,
                                                          const int32_t* __restrict__ order) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int64_t ray = order ? (int64_t)order[t] : t;
     RayState s;
-    state_from_rays(s0, n, ray, s);
+    state_from_rays(G, s0, n, ray, s);
     const StateView v = state_view(base, n);
     state_store(v, t, s);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v.v0[c][t] = s.v0[c];
+    for (int c = 0; c < 3; ++c) v.v0[c][t] = s0[(3 + c) * n + ray];
 }
 
 // Work unit = 64 consecutive (sorted) rays of one job: one wavefront.  The
@@ -168,11 +175,15 @@ __device__ void advance_unit(const slt_job& J, int64_t t) {
         DevGrid G;
         make_dev_grid_device(J.packed, &J.grid, &G);
         RayState s;
-        state_load(state_view(J.state, J.n_rays), t, s);
+        state_load(state_view(J.state, J.n_rays), t, G, s);
         const uint32_t before = s.steps;
         const bool finishing = J.step_end >= J.sched_steps;
         const int64_t ray = J.order ? (int64_t)J.order[t] : t;
-        SaveSink sink{finishing ? J.out_state : nullptr, J.out_tau, J.n_rays, ray, 0, J.n_seg};
+        // the state's v0 rows (m/s) follow rb[3], tau, path (f64) and off[3] (f32)
+        const float* v0m = reinterpret_cast<const float*>(static_cast<const char*>(J.state) +
+                                                          J.n_rays * (5 * 8 + 3 * 4));
+        SaveSink sink{finishing ? J.out_state : nullptr, J.out_tau, J.n_rays, ray, 0, J.n_seg,
+                      v0m, J.n_rays, t};
         integrate<KAPPA, true>(G, s, J.seg_nsub, J.seg_h, J.n_seg, J.step_begin, J.step_end,
                                sink);
         // Recompute the row's 14 addresses for the store instead of keeping the
@@ -309,16 +320,25 @@ extern "C" int64_t slt_ray_state_bytes(int64_t n_rays) {
 }
 
 extern "C" int slt_ray_state_init(void* state, int64_t n_rays, const float* s0,
-                                  const int32_t* order, slt_stream_t stream) {
+                                  const int32_t* order, const slt_grid_desc* g,
+                                  slt_stream_t stream) {
     clear_error();
-    if (n_rays < 0 || (n_rays > 0 && (!state || !s0))) {
+    if (n_rays < 0 || !g || (n_rays > 0 && (!state || !s0))) {
         set_error("slt_ray_state_init: invalid arguments");
         return SLT_ERR_INVALID;
     }
+    for (int a = 0; a < 3; ++a) {
+        if (g->n[a] < 2 || !(g->hi[a] > g->lo[a])) {
+            set_error("slt_ray_state_init: grid axis %d needs >= 2 points and hi > lo", a);
+            return SLT_ERR_INVALID;
+        }
+    }
     if (n_rays == 0) return SLT_OK;
+    DevGrid G = {};
+    fill_grid_units(g, &G);
     const int64_t blocks = (n_rays + 255) / 256;
     hipLaunchKernelGGL(state_init_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                       (hipStream_t)stream, state, n_rays, s0, order);
+                       (hipStream_t)stream, G, state, n_rays, s0, order);
     return check_launch("slt_ray_state_init");
 }
 

@@ -189,7 +189,8 @@ class ServingEngine:
             state = torch.empty(int(N.lib().slt_ray_state_bytes(n)), dtype=torch.uint8,
                                 device=self.dev)
             N.check(N.lib().slt_ray_state_init(state.data_ptr(), n, mat.rays.data_ptr(),
-                                               order.data_ptr(), N.stream_handle()))
+                                               order.data_ptr(), ctypes.byref(mat.grid.desc),
+                                               N.stream_handle()))
             seg_nsub = engine.to_device(nsub, self.dev)
             seg_h = engine.to_device(h.astype(np.float32), self.dev)
             out_state = torch.empty((6, n), dtype=torch.float32, device=self.dev)

@@ -31,7 +31,8 @@ def _setup(n=24, L=4e-3, seed=31, rays=3000, cl=10.0):
 
 
 def test_unpack_is_the_transpose_of_pack():
-    """<D(coeff ne), y> == <ne, D^T y coeff> for the gradient channels."""
+    """<P ne, y> == <ne, unpack(y)> for the gradient channels, P = the packed map
+    (coeff times the gradient stencil times 1 / cell size per axis)."""
     import torch
 
     from sloth_amd.adjoint import unpack
@@ -44,7 +45,7 @@ def test_unpack_is_the_transpose_of_pack():
     omega = 2 * np.pi * C / 532e-9
     for pd in ("z", "x"):
         g = engine.PackedGrid(ne, *coords, omega, probing_direction=pd)
-        planar = engine.grid_fields(ne, *coords, omega)
+        vox = g.voxel_view()
         y = torch.from_numpy(rng.standard_normal((*shape, 3)).astype(np.float32)).cuda()
         adj = torch.zeros(g.nbytes // 4, dtype=torch.float32, device="cuda")
         view = torch.as_strided(adj, (*shape, 4), (*[int(g.desc.stride[a]) * 4 for a in range(3)], 1))
@@ -52,7 +53,7 @@ def test_unpack_is_the_transpose_of_pack():
         ne_d = torch.from_numpy(ne).cuda()
         cfg = type("cfg", (), {"Te": None, "Z": None, "coulomb_log": None})
         dne = unpack(adj, g, ne_d, cfg).double().cpu().numpy()
-        lhs = sum(float((planar[a].double().cpu() * y[..., a].double().cpu()).sum())
+        lhs = sum(float((vox[..., a].double().cpu() * y[..., a].double().cpu()).sum())
                   for a in range(3))
         rhs = float((ne.astype(np.float64) * dne).sum())
         assert rhs == pytest.approx(lhs, rel=2e-5)

@@ -50,14 +50,20 @@ def test_pack_matches_oracle(orc, pd):
     g = engine.PackedGrid(ne, *coords, omega, probing_direction=pd, inv_brems=True, Te=te, Z=z)
     vox = g.voxel_view().cpu().numpy().astype(np.float64)
     c64 = [c.astype(np.float64) for c in coords]
+    # packed gradient channels are in grid units: times the cell size (hi - lo) / (n - 1)
+    cell = [(c[-1] - c[0]) / (len(c) - 1) for c in c64]
     grads = orc.gradients(ne.astype(np.float64), *c64, omega)
     kap = orc.kappa(ne.astype(np.float64), te.astype(np.float64), 1.5, omega)
     for ch, ref in enumerate([*grads, kap]):
+        got = vox[..., ch] * (cell[ch] if ch < 3 else 1.0)
         scale = np.abs(ref).max()
-        np.testing.assert_allclose(vox[..., ch], ref, rtol=1e-6, atol=1e-6 * scale)
-    planar = engine.grid_fields(ne, *coords, omega, Te=te, Z=z, want_kappa=True)
-    for ch, p in enumerate(planar):
-        np.testing.assert_array_equal(p.cpu().numpy(), vox[..., ch].astype(np.float32))
+        np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-6 * scale)
+    planar = [p.cpu().numpy() for p in
+              engine.grid_fields(ne, *coords, omega, Te=te, Z=z, want_kappa=True)]
+    for ch in range(3):   # the same fp64 gradient, rounded once per form
+        np.testing.assert_allclose(vox[..., ch], planar[ch] / cell[ch], rtol=2.5e-7,
+                                   atol=1e-30)
+    np.testing.assert_array_equal(planar[3], vox[..., 3].astype(np.float32))
     torch.cuda.synchronize()
 
 

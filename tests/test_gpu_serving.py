@@ -126,7 +126,7 @@ def test_advance_queues_drain_and_reset():
     order = engine.ray_order(mat.grid, mat.rays)
     state = torch.empty(int(L.slt_ray_state_bytes(n)), dtype=torch.uint8, device=dev)
     N.check(L.slt_ray_state_init(state.data_ptr(), n, mat.rays.data_ptr(), order.data_ptr(),
-                                 N.stream_handle()))
+                                 ctypes.byref(mat.grid.desc), N.stream_handle()))
     seg_nsub = torch.from_numpy(nsub).to(dev)
     seg_h = torch.from_numpy(h.astype(np.float32)).to(dev)
     out_state = torch.empty((6, n), dtype=torch.float32, device=dev)
