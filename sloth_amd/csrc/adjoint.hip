@@ -123,7 +123,7 @@ forward_ckpt_kernel(AdjArgs A) {
             const int64_t g1 = g0 + A.K < A.sched ? g0 + A.K : A.sched;
             integrate<KAPPA, false>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1, sink);
         }
-        nsteps = s.steps;
+        nsteps = inbox_steps(s);
     }
     if (A.total) add_steps(A.total, nsteps);
 }
@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
             s.tau = v.tau[t];
             s.fold = v.fold[t];
             s.path = 0.0;
-            s.steps = 0;
+            s.nout = 0;
             const int64_t g0 = (int64_t)c * A.K;
             const int64_t g1 = g0 + A.K < A.sched ? g0 + A.K : A.sched;
             integrate<KAPPA, false, GlobalTape>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1,

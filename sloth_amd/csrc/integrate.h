@@ -43,17 +43,40 @@ __device__ __forceinline__ float4 voxel(const SLT_GLOBAL slt_f4* __restrict__ p,
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+__device__ __forceinline__ float4 voxel_at(const SLT_GLOBAL char* p) {
+    const slt_f4 v = *reinterpret_cast<const SLT_GLOBAL slt_f4*>(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// The cell polynomial of corner voxel id (its 8 voxels).  A packed grid below
+// 4 GiB is addressed as one uniform base per corner plus the lane's 32-bit byte
+// offset, which each load takes as its VGPR offset: no 64-bit address
+// arithmetic per lane.  Larger grids take 64-bit addresses.
 __device__ __forceinline__ void load_cell(const DevGrid& G, int id, CellPoly& c) {
-    const SLT_GLOBAL slt_f4* __restrict__ p =
-        to_global(reinterpret_cast<const slt_f4*>(G.vox)) + id;
-    const float4 c000 = voxel(p, 0);
-    const float4 c001 = voxel(p, G.s2);
-    const float4 c010 = voxel(p, G.s1);
-    const float4 c011 = voxel(p, G.s1 + G.s2);
-    const float4 c100 = voxel(p, G.s0);
-    const float4 c101 = voxel(p, G.s0 + G.s2);
-    const float4 c110 = voxel(p, G.s0 + G.s1);
-    const float4 c111 = voxel(p, G.s0 + G.s1 + G.s2);
+    float4 c000, c001, c010, c011, c100, c101, c110, c111;
+    if (G.off32) {
+        const uint32_t off = (uint32_t)id << 4;
+        const SLT_GLOBAL char* b = reinterpret_cast<const SLT_GLOBAL char*>(to_global(G.vox));
+        c000 = voxel_at(b + off);
+        c001 = voxel_at(b + G.b2 + off);
+        c010 = voxel_at(b + G.b1 + off);
+        c011 = voxel_at(b + (G.b1 + G.b2) + off);
+        c100 = voxel_at(b + G.b0 + off);
+        c101 = voxel_at(b + (G.b0 + G.b2) + off);
+        c110 = voxel_at(b + (G.b0 + G.b1) + off);
+        c111 = voxel_at(b + (G.b0 + G.b1 + G.b2) + off);
+    } else {
+        const SLT_GLOBAL slt_f4* __restrict__ p =
+            to_global(reinterpret_cast<const slt_f4*>(G.vox)) + id;
+        c000 = voxel(p, 0);
+        c001 = voxel(p, G.s2);
+        c010 = voxel(p, G.s1);
+        c011 = voxel(p, G.s1 + G.s2);
+        c100 = voxel(p, G.s0);
+        c101 = voxel(p, G.s0 + G.s2);
+        c110 = voxel(p, G.s0 + G.s1);
+        c111 = voxel(p, G.s0 + G.s1 + G.s2);
+    }
     const float4 dz00 = f4sub(c001, c000), dz01 = f4sub(c011, c010);
     const float4 dz10 = f4sub(c101, c100), dz11 = f4sub(c111, c110);
     c.k0 = c000;
@@ -199,9 +222,12 @@ struct RayState {
     float dv[3];      // velocity change since the start
     double tau;       // optical depth
     double path;      // path length [m]
-    uint32_t steps;   // in-box RK4 steps
-    uint32_t fold;    // RK4 steps since the start (fold phase of off into rb)
+    uint32_t nout;    // RK4 steps with no stage point in the box
+    uint32_t fold;    // RK4 steps since the start (also the fold phase of off into rb)
 };
+
+// In-box RK4 steps of a ray so far.
+__device__ __forceinline__ uint32_t inbox_steps(const RayState& s) { return s.fold - s.nout; }
 
 // Start velocity component c in grid units/s (fp32), as the state holds it.
 __device__ __forceinline__ float velocity_units(const DevGrid& G, int c, float v) {
@@ -219,7 +245,7 @@ __device__ __forceinline__ void state_from_rays(const DevGrid& G, const float* _
     }
     s.tau = 0.0;
     s.path = 0.0;
-    s.steps = 0;
+    s.nout = 0;
     s.fold = 0;
 }
 
@@ -331,9 +357,11 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                                             fmaf(h, v3z, rh[2]), cc, A4);
                 // r += h v + h^2/6 (A1 + A2 + A3)   (= h/6 (v1 + 2 v2 + 2 v3 + v4))
                 // v += h/6 (A1 + 2 A2 + 2 A3 + A4)
-                const float dx = fmaf(h, vh[0], hh6 * (A1.x + A2.x + A3.x));
-                const float dy = fmaf(h, vh[1], hh6 * (A1.y + A2.y + A3.y));
-                const float dz = fmaf(h, vh[2], hh6 * (A1.z + A2.z + A3.z));
+                // A2 + A3 enters both updates
+                const float sx = A2.x + A3.x, sy = A2.y + A3.y, sz = A2.z + A3.z;
+                const float dx = fmaf(h, vh[0], hh6 * (A1.x + sx));
+                const float dy = fmaf(h, vh[1], hh6 * (A1.y + sy));
+                const float dz = fmaf(h, vh[2], hh6 * (A1.z + sz));
                 s.off[0] += dx;
                 s.off[1] += dy;
                 s.off[2] += dz;
@@ -342,11 +370,13 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                     const float ex = dx * G.sp0, ey = dy * G.sp1, ez = dz * G.sp2;
                     s.path += (double)__builtin_amdgcn_sqrtf(fmaf(ex, ex, fmaf(ey, ey, ez * ez)));
                 }
-                s.dv[0] += h6 * (A1.x + 2.f * (A2.x + A3.x) + A4.x);
-                s.dv[1] += h6 * (A1.y + 2.f * (A2.y + A3.y) + A4.y);
-                s.dv[2] += h6 * (A1.z + 2.f * (A2.z + A3.z) + A4.z);
+                s.dv[0] += h6 * (A1.x + 2.f * sx + A4.x);
+                s.dv[1] += h6 * (A1.y + 2.f * sy + A4.y);
+                s.dv[2] += h6 * (A1.z + 2.f * sz + A4.z);
                 if (KAPPA) s.tau += (double)(h6 * (A1.w + 2.f * (A2.w + A3.w) + A4.w));
-                s.steps += in ? 1u : 0u;
+                // every RK4 step advances fold; the rare step with no stage point in the
+                // box is counted on its own (in-box steps = fold - nout)
+                if (__builtin_expect(!in, 0)) ++s.nout;
                 ++k;
                 if ((++s.fold & 7u) == 0) {
 #pragma unroll

@@ -29,6 +29,8 @@ struct DevGrid {
     const float4* __restrict__ vox;
     int n0, n1, n2;
     int s0, s1, s2;
+    int64_t b0, b1, b2;   // strides in bytes
+    int off32;            // packed grid below 4 GiB: byte offsets fit in 32 bits
     float ub0, ub1, ub2;  // upper box face in grid units: n - 1 (the lower one is 0)
     float sp0, sp1, sp2;  // cell size [m] (path length in fp32)
     double org[3];        // grid origin lo [m]
@@ -49,6 +51,13 @@ __host__ __device__ inline void fill_grid_units(const slt_grid_desc* g, DevGrid*
     }
     d->ub0 = (float)(n[0] - 1); d->ub1 = (float)(n[1] - 1); d->ub2 = (float)(n[2] - 1);
     d->sp0 = (float)d->dsp[0]; d->sp1 = (float)d->dsp[1]; d->sp2 = (float)d->dsp[2];
+}
+
+// Byte strides and the 32-bit addressing flag of a DevGrid.
+__host__ __device__ inline void fill_grid_strides(const slt_grid_desc* g, DevGrid* d) {
+    d->b0 = g->stride[0] * 16; d->b1 = g->stride[1] * 16; d->b2 = g->stride[2] * 16;
+    const int64_t total = (int64_t)g->n[0] * g->n[1] * g->n[2];
+    d->off32 = total * 16 <= (int64_t)0xFFFFFFFF ? 1 : 0;
 }
 
 // 1 / cell size of axis a, as fill_grid_units computes it (the pack's scale).
@@ -80,6 +89,7 @@ inline int make_dev_grid(const void* packed, const slt_grid_desc* g, DevGrid* d)
     d->vox = static_cast<const float4*>(packed);
     d->n0 = g->n[0]; d->n1 = g->n[1]; d->n2 = g->n[2];
     d->s0 = (int)g->stride[0]; d->s1 = (int)g->stride[1]; d->s2 = (int)g->stride[2];
+    fill_grid_strides(g, d);
     fill_grid_units(g, d);
     d->nchan = g->nchan;
     return SLT_OK;
@@ -92,6 +102,7 @@ __host__ __device__ inline void make_dev_grid_device(const void* packed, const s
     d->vox = static_cast<const float4*>(packed);
     d->n0 = g->n[0]; d->n1 = g->n[1]; d->n2 = g->n[2];
     d->s0 = (int)g->stride[0]; d->s1 = (int)g->stride[1]; d->s2 = (int)g->stride[2];
+    fill_grid_strides(g, d);
     fill_grid_units(g, d);
     d->nchan = g->nchan;
 }

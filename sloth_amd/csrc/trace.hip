@@ -77,15 +77,15 @@ trace_kernel(TraceK K) {
         for (int q = 0; q < K.nseg; ++q) total += K.seg_nsub[q];
         integrate<KAPPA, PATH>(K.G, s, K.seg_nsub, K.seg_h, K.nseg, 0, total, sink);
         if (PATH && K.path) K.path[ray] = s.path;
-        if (K.steps) K.steps[ray] = s.steps;
-        nsteps = s.steps;
+        nsteps = inbox_steps(s);
+        if (K.steps) K.steps[ray] = nsteps;
     }
     if (K.total) add_steps(K.total, nsteps);
 }
 
 // ---------------------------------------------------------------------------
 // Resumable ray state (serving): structure-of-arrays in one buffer.
-//   rb[3] f64 | tau f64 | path f64 | off[3] f32 | v0[3] f32 | dv[3] f32 | steps u32 | fold u32
+//   rb[3] f64 | tau f64 | path f64 | off[3] f32 | v0[3] f32 | dv[3] f32 | nout u32 | fold u32
 // rb, off and dv are in the grid units of the job's grid; v0 is the start
 // velocity in m/s (converted on load, and the base of the saved velocity).
 // ---------------------------------------------------------------------------
@@ -96,7 +96,7 @@ struct StateView {
     SLT_GLOBAL float* off[3];
     SLT_GLOBAL float* v0[3];
     SLT_GLOBAL float* dv[3];
-    SLT_GLOBAL uint32_t* steps;
+    SLT_GLOBAL uint32_t* nout;
     SLT_GLOBAL uint32_t* fold;
 };
 
@@ -115,7 +115,7 @@ __device__ __forceinline__ StateView state_view(void* base, int64_t n) {
         v.dv[c] = f + (6 + c) * n;
     }
     SLT_GLOBAL uint32_t* u = reinterpret_cast<SLT_GLOBAL uint32_t*>(f + 9 * n);
-    v.steps = u;
+    v.nout = u;
     v.fold = u + n;
     return v;
 }
@@ -131,7 +131,7 @@ __device__ __forceinline__ void state_load(const StateView& v, int64_t i, const 
     }
     s.tau = v.tau[i];
     s.path = v.path[i];
-    s.steps = v.steps[i];
+    s.nout = v.nout[i];
     s.fold = v.fold[i];
 }
 
@@ -144,7 +144,7 @@ __device__ __forceinline__ void state_store(const StateView& v, int64_t i, const
     }
     v.tau[i] = s.tau;
     v.path[i] = s.path;
-    v.steps[i] = s.steps;
+    v.nout[i] = s.nout;
     v.fold[i] = s.fold;
 }
 
@@ -176,7 +176,7 @@ __device__ void advance_unit(const slt_job& J, int64_t t) {
         make_dev_grid_device(J.packed, &J.grid, &G);
         RayState s;
         state_load(state_view(J.state, J.n_rays), t, G, s);
-        const uint32_t before = s.steps;
+        const uint32_t before = inbox_steps(s);
         const bool finishing = J.step_end >= J.sched_steps;
         const int64_t ray = J.order ? (int64_t)J.order[t] : t;
         // the state's v0 rows (m/s) follow rb[3], tau, path (f64) and off[3] (f32)
@@ -193,9 +193,9 @@ __device__ void advance_unit(const slt_job& J, int64_t t) {
         state_store(state_view(J.state, J.n_rays), row, s);
         if (finishing) {
             if (J.out_path) J.out_path[ray] = s.path;
-            if (J.out_steps) J.out_steps[ray] = s.steps;
+            if (J.out_steps) J.out_steps[ray] = inbox_steps(s);
         }
-        nsteps = s.steps - before;
+        nsteps = inbox_steps(s) - before;
     }
     if (J.step_counter) add_steps(J.step_counter, nsteps);
 }
