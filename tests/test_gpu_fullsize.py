@@ -113,3 +113,31 @@ def test_c3_turbulent_512_oracle_subset_and_invariants(orc):
     ref = oracle_trace(orc, ne, coords, s0[:, idx], gpu["nsub"], gpu["h"], lwl=532e-9,
                        inv_brems=True, Te=20.0, Z=1.0, coulomb_log=10.0)
     print("C3 subset parity", assert_parity(_subset(gpu, idx), ref, coords))
+
+
+def test_trough_646_packed_grid_above_4gib_closed_form():
+    """A packed grid above 4 GiB (646^3 x 16 B) takes 64-bit corner addresses; the
+    quadratic trough's closed form pins every ray there as on the smaller grids."""
+    import torch
+
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import quadratic_trough
+
+    L = 12e-3
+    n = 646
+    assert n ** 3 * 16 > 0xFFFFFFFF
+    ne, coords = quadratic_trough(n, L)
+    s0 = Beam(20000, 4e-3, 0.0, L / 2, beam_type="square", seeded=True).s0
+    t = np.linspace(0, 11e-3, 3) / C
+    gpu = gpu_trace(ne, coords, s0, t, lwl=532e-9)
+    assert gpu["grid"].nbytes > 0xFFFFFFFF
+    del ne
+    omega = 2 * np.pi * C / 532e-9
+    w = C * np.sqrt(5e24 / (3.14207787e-4 * omega ** 2)) / 3e-3
+    x0, y0 = s0[0].astype(np.float64), s0[1].astype(np.float64)
+    for k in (1, 2):
+        st = gpu["states"][k]
+        np.testing.assert_allclose(st[0], x0 * np.cos(w * t[k]), rtol=0, atol=2e-9)
+        np.testing.assert_allclose(st[1], y0 * np.cos(w * t[k]), rtol=0, atol=2e-9)
+    del gpu
+    torch.cuda.empty_cache()
