@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SLT_ABI_VERSION 1
+#define SLT_ABI_VERSION 2
 
 #define SLT_OK 0
 #define SLT_ERR_INVALID (-1)     /* bad argument (shape, pointer, range) */

@@ -33,7 +33,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert not missing, missing
     assert set(N.EXPORTED) == set(declared)
     L = N.lib()
-    assert L.slt_version() == 1
+    assert L.slt_version() == 2
     assert L.slt_last_error() in (b"", None) or isinstance(L.slt_last_error(), bytes)
 
 
