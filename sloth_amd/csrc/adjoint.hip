@@ -202,28 +202,25 @@ __device__ __forceinline__ void scatter_corners(const DevGrid& G, float* __restr
 }
 
 // Where flushed moments go: the voxel adjoint in global memory and this
-// wavefront's flush slots in LDS (kFlushSlots cells x 32 floats).
+// wavefront's flush area in LDS: one row of 32 moments per lane (8 float4: m0,
+// mx, my, mz, mxy, mxz, myz, mxyz; rows 16-byte aligned), then kFlushSlots
+// cell sums of 32 floats (padded so that the slots start in different banks),
+// then the slots' cell ids.
 constexpr int kFlushSlots = 16;
+constexpr int kRowStride = 36;
+constexpr int kSlotStride = 33;
+constexpr int kWaveLds = 64 * kRowStride + kFlushSlots * kSlotStride + kFlushSlots;
+static_assert(kRowStride % 4 == 0 && kWaveLds % 4 == 0, "rows must stay 16-byte aligned");
 #define SLT_LDS __attribute__((address_space(3)))
 
 struct AdjOut {
     float* __restrict__ adj;
     SLT_LDS float* lds;
+    __device__ __forceinline__ SLT_LDS float* sums() const { return lds + 64 * kRowStride; }
+    __device__ __forceinline__ SLT_LDS int* ids() const {
+        return reinterpret_cast<SLT_LDS int*>(lds + 64 * kRowStride + kFlushSlots * kSlotStride);
+    }
 };
-
-__device__ __forceinline__ void lds_add4(SLT_LDS float* p, const float4& v) {
-    __hip_atomic_fetch_add(p + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    __hip_atomic_fetch_add(p + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    __hip_atomic_fetch_add(p + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    __hip_atomic_fetch_add(p + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-}
-
-// Reads a float4 of a slot and leaves zeros behind.
-__device__ __forceinline__ float4 lds_take4(SLT_LDS float* p) {
-    const float4 v = make_float4(p[0], p[1], p[2], p[3]);
-    p[0] = p[1] = p[2] = p[3] = 0.f;
-    return v;
-}
 
 // Orders this wavefront's LDS accesses across its lanes.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -232,53 +229,83 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The adjoint of corner E (bits: 4 = x, 2 = y, 1 = z) of a cell from the 8
+// moments of one channel (m0, mx, my, mz, mxy, mxz, myz, mxyz): corner E has
+// weight prod_a (E_a ? t_a : 1 - t_a), so its adjoint is
+// sum over monomials S containing E of (-1)^|S \ E| m_S.
+__device__ __forceinline__ float corner_adjoint(int E, const float m[8]) {
+    constexpr int kSet[8] = {0, 4, 2, 1, 6, 5, 3, 7};
+    float v = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int S = kSet[q];
+        const float w = (S & E) == E ? ((__popc(S ^ E) & 1) ? -1.f : 1.f) : 0.f;
+        v = fmaf(w, m[q], v);
+    }
+    return v;
+}
+
 // Collective flush (whole wavefront, uniform control flow): the lanes with
 // `need` set give their moments to their cell.  A sorted wavefront holds a few
-// cells at a time, and at a flush nearly all its lanes leave theirs together,
-// so each distinct cell gets an LDS slot (the first lane still pending leads
-// it), every lane adds its moments to its slot with LDS atomics, and each
-// slot's leader reads the sum back and scatters the cell's corner adjoints
-// with global atomics: one lane's atomics per cell, with no cross-lane
-// reduction per cell.  Lanes of cells beyond the slots scatter their own.
+// cells at a time, and at a flush nearly all its lanes leave theirs together.
+// The flushing lanes store their moments in their LDS rows; then, cell by cell,
+// the wavefront sums the rows of the cell's lanes (component k in lanes k and
+// k + 32, which take alternate members) into a slot; then the whole wavefront
+// scatters the slots' corner adjoints: entry e = (slot, corner, channel) goes to
+// lane e mod 64, so a flush of c cells issues ceil(32 c / 64) full-width atomic
+// instructions.  No LDS atomics: float atomics on LDS addresses shared by many
+// lanes serialise.  Lanes of cells beyond the slots scatter their own.
 __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, Moments& M,
                                            bool need) {
-    uint64_t pending = __ballot(need);
     const int lane = __lane_id();
-    int slot = -1;  // the slot this lane adds to
-    int led = -1;   // the slot this lane reads back and scatters
-    for (int g = 0; pending && g < kFlushSlots; ++g) {
+    if (need) {
+        SLT_LDS slt_f4* row = reinterpret_cast<SLT_LDS slt_f4*>(out.lds + lane * kRowStride);
+        const float4* src[8] = {&M.m0, &M.mx, &M.my, &M.mz, &M.mxy, &M.mxz, &M.myz, &M.mxyz};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) row[q] = slt_f4{src[q]->x, src[q]->y, src[q]->z, src[q]->w};
+    }
+    wave_lds_sync();
+    const bool upper = lane >= 32;
+    const int k = lane & 31;
+    uint64_t pending = __ballot(need);
+    int nslots = 0;
+    while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const int key = __builtin_amdgcn_readlane(M.id, leader);
         const bool mine = need && M.id == key;
-        pending &= ~__ballot(mine);
-        if (mine) slot = g;
-        if (lane == leader) led = g;
-    }
-    if (need && slot < 0) scatter_corners(G, out.adj, M.id, M);
-    if (slot >= 0) {
-        SLT_LDS float* p = out.lds + slot * 32;
-        lds_add4(p + 0, M.m0);
-        lds_add4(p + 4, M.mx);
-        lds_add4(p + 8, M.my);
-        lds_add4(p + 12, M.mz);
-        lds_add4(p + 16, M.mxy);
-        lds_add4(p + 20, M.mxz);
-        lds_add4(p + 24, M.myz);
-        lds_add4(p + 28, M.mxyz);
+        uint64_t mem = __ballot(mine);
+        pending &= ~mem;
+        if (nslots < kFlushSlots) {
+            float acc = 0.f;
+            while (mem) {
+                const int m1 = __ffsll((unsigned long long)mem) - 1;
+                mem &= mem - 1;
+                const int m2 = mem ? __ffsll((unsigned long long)mem) - 1 : m1;
+                const bool two = mem != 0;
+                mem &= mem - 1;
+                const float v = out.lds[(upper ? m2 : m1) * kRowStride + k];
+                acc += (upper && !two) ? 0.f : v;
+            }
+            acc += __shfl_xor(acc, 32);
+            if (!upper) out.sums()[nslots * kSlotStride + k] = acc;
+            if (lane == 0) out.ids()[nslots] = key;
+            ++nslots;
+        } else if (mine) {
+            scatter_corners(G, out.adj, M.id, M);
+        }
     }
     wave_lds_sync();
-    if (led >= 0) {
-        SLT_LDS float* p = out.lds + led * 32;
-        Moments S;
-        S.m0 = lds_take4(p + 0);
-        S.mx = lds_take4(p + 4);
-        S.my = lds_take4(p + 8);
-        S.mz = lds_take4(p + 12);
-        S.mxy = lds_take4(p + 16);
-        S.mxz = lds_take4(p + 20);
-        S.myz = lds_take4(p + 24);
-        S.mxyz = lds_take4(p + 28);
-        scatter_corners(G, out.adj, M.id, S);
+    const int nent = nslots * 32;
+    for (int e = lane; e < nent; e += 64) {
+        const int sl = e >> 5, corner = (e >> 2) & 7, ch = e & 3;
+        const SLT_LDS float* p = out.sums() + sl * kSlotStride + ch;
+        float m[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) m[q] = p[4 * q];
+        const float v = corner_adjoint(corner, m);
+        const int off =
+            ((corner >> 2) & 1) * G.s0 + ((corner >> 1) & 1) * G.s1 + (corner & 1) * G.s2;
+        atomicAdd(out.adj + (int64_t)(out.ids()[sl] + off) * 4 + ch, v);
     }
     wave_lds_sync();
 }
@@ -412,12 +439,10 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
     const int64_t n = A.n;
     const int64_t tt = valid ? t : 0;  // in-bounds row for the lanes past the end
     const int64_t ray = valid ? (A.order ? (int64_t)A.order[t] : t) : 0;
-    // this wavefront's flush slots, zero between flushes
-    __shared__ float s_flush[(256 / 64) * kFlushSlots * 32];
+    // this wavefront's flush area (rows are written as 16-byte vectors)
+    __shared__ __attribute__((aligned(16))) float s_flush[(256 / 64) * kWaveLds];
     const AdjOut out{A.grid_adj,
-                     (SLT_LDS float*)s_flush + (threadIdx.x / 64) * (kFlushSlots * 32)};
-    for (int i = __lane_id(); i < kFlushSlots * 32; i += 64) out.lds[i] = 0.f;
-    wave_lds_sync();
+                     (SLT_LDS float*)s_flush + (threadIdx.x / 64) * kWaveLds};
     float rb[3] = {0.f, 0.f, 0.f}, vb[3] = {0.f, 0.f, 0.f}, v0[3] = {0.f, 0.f, 0.f};
     float tb = 0.f;
     if (valid) {
