@@ -41,18 +41,42 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles():
-    """HBM bytes per trace launch measured with rocprofv3 --pmc (committed summary), or None."""
-    p = os.path.join(ROOT, "profiles", "trace_kernel_turbulent512_pmc.md")
+PMC_SUMMARY = os.path.join("profiles", "trace_kernel_turbulent512_pmc.md")
+
+
+def profile_value(key):
+    """A figure of the committed rocprofv3 --pmc summary of the trace kernel (or None)."""
+    p = os.path.join(ROOT, PMC_SUMMARY)
     if not os.path.exists(p):
         return None
     for line in open(p):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             try:
                 return float(line.split(":", 1)[1].strip())
             except ValueError:
                 return None
     return None
+
+
+def traffic_from_profiles():
+    """HBM bytes per trace launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE)."""
+    return profile_value("traffic_bytes_per_launch")
+
+
+def valu_roofline(steps_per_s_kernel, n_cu):
+    """VALU issue of the trace kernel, its binding resource: wave64 VALU instructions per
+    SIMD-cycle against the 0.5 a SIMD issues.  The instructions per ray-step and the clock
+    under load are PMC figures (SQ_INSTS_VALU, GRBM_GUI_ACTIVE) of the committed summary;
+    the ray-steps per second are this run's."""
+    lane_instr = profile_value("valu_lane_instr_per_ray_step")
+    ghz = profile_value("sclk_ghz_under_load")
+    if not lane_instr or not ghz or not n_cu:
+        return None
+    achieved = steps_per_s_kernel * lane_instr / 64.0 / (4 * n_cu * ghz * 1e9)
+    return {"bound": "valu_issue", "achieved": round(achieved, 4), "peak": 0.5,
+            "unit": "wave64 VALU instructions per SIMD-cycle", "frac": round(achieved / 0.5, 4),
+            "lane_instr_per_ray_step": lane_instr, "clock_ghz": ghz, "simds": 4 * n_cu,
+            "source": PMC_SUMMARY}
 
 
 def cpu_baseline(ne, coords, s0, nsub, h, omega, target_s):
@@ -293,6 +317,8 @@ def main():
                 "traffic": traffic_from_profiles(),
                 "kernel": "slt::trace_kernel<true,true>", "kernel_ms": round(kern_ms, 3),
                 "algorithmic_bytes_per_launch": total_steps * BYTES_PER_STEP}
+    valu = valu_roofline(total_steps / (kern_ms * 1e-3),
+                         torch.cuda.get_device_properties(0).multi_processor_count)
     log(f"[bench] {ms_per_step:.2f} ms/step, trace kernel {kern_ms:.2f} ms, "
         f"{value:.3e} ray-steps/s, detector sum {int(counts.sum())}")
     cpu = None
@@ -333,6 +359,7 @@ def main():
                    "accumulators": "fp64 tau and path"},
         "rhs_evals_per_s": 4.0 * value,   # RK4: 4 RHS evaluations per ray-step
         "roofline": roofline,
+        "valu_roofline": valu,
         "cpu_baseline": cpu,
         "serving": serving,
         "adjoint": adjoint,
