@@ -67,6 +67,38 @@ def test_pack_matches_oracle(orc, pd):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("case", ["scalar_te", "scalar_te_cl", "no_kappa", "long_axis0",
+                                  "axis0_at_table_max"])
+def test_pack_kappa_modes_and_axis_lengths(orc, case):
+    """k1 against the oracle for each kappa mode of the column-marching pack (scalar Te/Z
+    with and without a Coulomb-log override, no kappa), across its 32-plane run boundaries
+    (70 planes), at the longest axis 0 its LDS table takes (2048) and beyond it (2100
+    planes: the per-point kernel).  Densities reach 3x critical at 532 nm, so the
+    plasma-frequency branch of the Coulomb logarithm runs too."""
+    from sloth_amd.core import engine
+
+    shape = {"long_axis0": (2100, 3, 4), "axis0_at_table_max": (2048, 2, 3)}.get(case,
+                                                                               (70, 9, 11))
+    rng = np.random.default_rng(11)
+    omega = 2 * np.pi * C / 532e-9
+    n_cr = engine.critical_density(omega)
+    ne = rng.uniform(1e-3 * n_cr, 3.0 * n_cr, shape).astype(np.float32)
+    coords = [np.float32(np.linspace(-a, a, n)) for a, n in zip((4e-3, 1e-3, 2e-3), shape)]
+    inv = case != "no_kappa"
+    cl = 10.0 if case == "scalar_te_cl" else None
+    g = engine.PackedGrid(ne, *coords, omega, inv_brems=inv, Te=20.0 if inv else None,
+                          Z=1.0 if inv else None, coulomb_log=cl)
+    vox = g.voxel_view().cpu().numpy().astype(np.float64)
+    c64 = [c.astype(np.float64) for c in coords]
+    cell = [(c[-1] - c[0]) / (len(c) - 1) for c in c64]
+    refs = list(orc.gradients(ne.astype(np.float64), *c64, omega))
+    if inv:
+        refs.append(orc.kappa(ne.astype(np.float64), 20.0, 1.0, omega, coulomb_log=cl))
+    for ch, ref in enumerate(refs):
+        got = vox[..., ch] * (cell[ch] if ch < 3 else 1.0)
+        np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-6 * np.abs(ref).max())
+
+
 def test_kappa_kernel_matches_oracle(orc):
     from sloth_amd.core.propagator import kappa_inv_brems
 
