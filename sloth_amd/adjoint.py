@@ -68,8 +68,8 @@ class _TraceFn(torch.autograd.Function):
         sched = int(nsub.astype(np.int64).sum())
         K = int(cfg.ckpt_every)
         order = engine.ray_order(grid, rays) if cfg.sort and n > 0 else None
-        nsub_d = engine.to_device(nsub, dev)
-        h_d = engine.to_device(h.astype(np.float32), dev)
+        nsub_d = engine.const_device(nsub, dev)
+        h_d = engine.const_device(h.astype(np.float32), dev)
         ck = torch.empty(max(int(L.slt_checkpoint_bytes(n, sched, K)), 1), dtype=torch.uint8,
                          device=dev)
         state = torch.empty((6, n), dtype=torch.float32, device=dev)
@@ -115,7 +115,7 @@ class _TraceFn(torch.autograd.Function):
 def unpack(adj: torch.Tensor, grid: engine.PackedGrid, ne_d: torch.Tensor, cfg) -> torch.Tensor:
     """dL/dn_e (nx, ny, nz) from a packed voxel adjoint."""
     dev = ne_d.device
-    cd = [engine.to_device(c, dev) for c in grid.coords]
+    cd = [engine.const_device(c, dev) for c in grid.coords]
     te_d = z_d = None
     if grid.inv_brems:
         te_d = engine.device_f32(cfg.Te, dev).reshape(-1)

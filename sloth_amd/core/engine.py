@@ -9,6 +9,7 @@ memory and the stream.
 from __future__ import annotations
 
 import ctypes
+from collections import OrderedDict
 from dataclasses import dataclass
 
 import numpy as np
@@ -61,6 +62,51 @@ def to_device(a, device, stream=None) -> torch.Tensor:
         return t.to(device)
 
 
+_CONST: "OrderedDict[tuple, tuple]" = OrderedDict()  # key -> (tensor, upload event)
+_CONST_MAX = 512
+
+
+def const_device(a, device, stream=None) -> torch.Tensor:
+    """A read-only device copy of a small host array, shared by every caller that passes
+    the same dtype, shape and bytes.
+
+    For coordinates, step schedules, bin edges and scalar Te/Z. Requests of a serving mix
+    reuse a handful of these, and each small host-to-device copy is a blit kernel of tens
+    of microseconds. Callers must not write to the result. Arrays above 1 MiB are
+    copied as ``to_device`` does. The upload's completion is recorded as an event. A
+    later hit, on whatever stream, makes that stream wait for the event (a GPU-side
+    dependency, no host sync). An entry stays alive until evicted. Eviction first
+    waits for the device, so a kernel queued on any stream has finished reading it.
+    """
+    arr = np.ascontiguousarray(a)
+    if arr.nbytes > _STAGED_MAX_BYTES:
+        return to_device(arr, device, stream)
+    dev = torch.device(device)
+    key = (dev.type, dev.index, arr.dtype.str, arr.shape, arr.tobytes())
+    use = stream if stream is not None else torch.cuda.current_stream(dev)
+    hit = _CONST.get(key)
+    if hit is not None:
+        _CONST.move_to_end(key)
+        t, uploaded = hit
+        use.wait_event(uploaded)
+        return t
+    t = to_device(arr, dev, use)
+    uploaded = torch.cuda.Event()
+    uploaded.record(use)
+    _CONST[key] = (t, uploaded)
+    if len(_CONST) > _CONST_MAX:
+        torch.cuda.synchronize(dev)
+        _CONST.popitem(last=False)
+    return t
+
+
+def _small_f32(a, device, stream=None) -> torch.Tensor:
+    """device_f32 for read-only inputs: host values go through const_device."""
+    if isinstance(a, torch.Tensor):
+        return device_f32(a, device, stream)
+    return const_device(host_f32(a), device, stream)
+
+
 def device_f32(a, device, stream=None) -> torch.Tensor:
     if isinstance(a, torch.Tensor):
         return a.detach().to(device=device, dtype=torch.float32).contiguous()
@@ -101,13 +147,13 @@ class PackedGrid:
         self.shape = (nx, ny, nz)
         self.nbytes = int(L.slt_grid_bytes(ctypes.byref(desc)))
         self.voxels = torch.empty(self.nbytes // 4, dtype=torch.float32, device=dev)
-        cd = [to_device(c, dev, stream) for c in self.coords]
+        cd = [const_device(c, dev, stream) for c in self.coords]
         te_d = z_d = None
         if self.inv_brems:
             if Te is None or Z is None:
                 raise AssertionError("Te and Z must both be provided when inv_brems=True.")
-            te_d = device_f32(Te, dev, stream).reshape(-1)
-            z_d = device_f32(Z, dev, stream).reshape(-1)
+            te_d = _small_f32(Te, dev, stream).reshape(-1)
+            z_d = _small_f32(Z, dev, stream).reshape(-1)
         s = N.stream_handle(stream)
         N.check(L.slt_grid_pack(
             ne_d.data_ptr(), cd[0].data_ptr(), cd[1].data_ptr(), cd[2].data_ptr(),
@@ -145,7 +191,7 @@ def grid_fields(ne, x, y, z, omega, *, Te=None, Z=None, coulomb_log=None, want_k
                                  ctypes.byref(desc)))
     outs = [torch.empty((nx, ny, nz), dtype=torch.float32, device=dev)
             for _ in range(4 if want_kappa else 3)]
-    cd = [to_device(c, dev) for c in coords]
+    cd = [const_device(c, dev) for c in coords]
     te_d = z_d = None
     if want_kappa:
         te_d = device_f32(Te, dev).reshape(-1)
@@ -259,8 +305,8 @@ def trace(grid: PackedGrid, s0, save_times, *, step_fraction: float = 0.5,
         return TraceResult(states, tau, path, steps, total, None, nsub, h)
     if order is None and sort:
         order = ray_order(grid, rays, stream)
-    seg_nsub = to_device(nsub, dev, stream)
-    seg_h = to_device(h.astype(np.float32), dev, stream)
+    seg_nsub = const_device(nsub, dev, stream)
+    seg_h = const_device(h.astype(np.float32), dev, stream)
     a = N.TraceArgs()
     a.n_rays = n
     a.s0 = rays.data_ptr()

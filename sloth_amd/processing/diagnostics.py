@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from .. import _native as N
-from ..core.engine import to_device
+from ..core.engine import const_device, to_device
 
 
 def m_to_mm(r):
@@ -233,8 +233,8 @@ def histogram2d_device(x: torch.Tensor, y: torch.Tensor, xedges: np.ndarray, yed
         raise TypeError("x and y must both be float32 or both float64")
     dev = x.device
     nxb, nyb = len(xedges) - 1, len(yedges) - 1
-    xe = to_device(np.asarray(xedges, dtype=np.float64), dev)
-    ye = to_device(np.asarray(yedges, dtype=np.float64), dev)
+    xe = const_device(np.asarray(xedges, dtype=np.float64), dev)
+    ye = const_device(np.asarray(yedges, dtype=np.float64), dev)
     counts = torch.zeros((nyb, nxb), dtype=torch.int32, device=dev)
     wsum = None
     w = None

@@ -191,8 +191,8 @@ class ServingEngine:
             N.check(N.lib().slt_ray_state_init(state.data_ptr(), n, mat.rays.data_ptr(),
                                                order.data_ptr(), ctypes.byref(mat.grid.desc),
                                                N.stream_handle()))
-            seg_nsub = engine.to_device(nsub, self.dev)
-            seg_h = engine.to_device(h.astype(np.float32), self.dev)
+            seg_nsub = engine.const_device(nsub, self.dev)
+            seg_h = engine.const_device(h.astype(np.float32), self.dev)
             out_state = torch.empty((6, n), dtype=torch.float32, device=self.dev)
             out_tau = torch.zeros(n, dtype=torch.float64, device=self.dev)
             out_path = torch.empty(n, dtype=torch.float64, device=self.dev)

@@ -73,7 +73,7 @@ def field_on_device(spec: FieldSpec, device) -> torch.Tensor:
     """(n, n, n) float32 electron density [m^-3] built on the device."""
     n = spec.side
     if spec.kind == "sine":
-        x = engine.to_device(_coords(spec)[0].astype(np.float64), device)
+        x = engine.const_device(_coords(spec)[0].astype(np.float64), device)
         prof = spec.n0 * (1.0 + spec.frac * torch.sin(2 * math.pi * x / spec.wavelength))
         return prof.to(torch.float32)[:, None, None].expand(n, n, n).contiguous()
     if spec.kind != "kolmogorov":

@@ -85,3 +85,31 @@ def test_trace_rays_that_miss_the_grid():
     end = r.states[0].cpu().numpy()
     np.testing.assert_allclose(end[2], s0[2] + s0[5] * t[-1], rtol=1e-6)
     np.testing.assert_allclose(r.path.cpu().numpy(), 2 * L, rtol=1e-6)
+
+
+def test_const_device_shares_reads_across_streams_and_evicts():
+    """Small read-only uploads are shared by content; a hit on another stream waits for the
+    upload; entries past the cap are evicted and new ones still hold the right values."""
+    import torch
+
+    from sloth_amd.core import engine
+
+    dev = torch.device("cuda", 0)
+    a = np.linspace(-1.0, 1.0, 257).astype(np.float32)
+    side = torch.cuda.Stream(dev)
+    t1 = engine.const_device(a, dev, side)
+    t2 = engine.const_device(a.copy(), dev)          # same bytes: the same tensor
+    assert t1 is t2
+    t3 = engine.const_device(a.astype(np.float64), dev)
+    assert t3 is not t1 and t3.dtype == torch.float64
+    with torch.cuda.stream(side):
+        out = (t2 * 2.0).cpu().numpy()               # read on the side stream
+    np.testing.assert_array_equal(out, 2.0 * a)
+    torch.cuda.synchronize()
+    # fill past the cap: eviction waits for the device, later uploads are exact
+    for i in range(engine._CONST_MAX + 8):
+        v = np.full(3, float(i), np.float64)
+        np.testing.assert_array_equal(engine.const_device(v, dev).cpu().numpy(), v)
+    assert len(engine._CONST) <= engine._CONST_MAX
+    big = np.zeros((1 << 20) // 4 + 1, np.float32)   # above 1 MiB: an ordinary copy
+    assert engine.const_device(big, dev) is not engine.const_device(big, dev)
