@@ -69,6 +69,37 @@ def _coords(spec: FieldSpec):
     return c, c.copy(), c.copy()
 
 
+_AMP: dict = {}  # (n, length, l_min, device) -> (spectrum, event recorded after building it)
+_AMP_MAX = 8
+
+
+def _spectrum_amp(n: int, length: float, l_min: float, device) -> torch.Tensor:
+    """|k|^(-11/6) on the band 2 pi / length <= |k| <= 2 pi / l_min of the (n, n, n / 2 + 1)
+    half spectrum, zero elsewhere.  It depends only on the grid, so it is built once per
+    (n, length, l_min) and shared (read-only) by every request of that shape."""
+    key = (n, float(length), float(l_min), str(device))
+    use = torch.cuda.current_stream(device)
+    hit = _AMP.get(key)
+    if hit is not None:
+        amp, built = hit
+        use.wait_event(built)   # built on another stream: wait for it on the GPU
+        return amp
+    dx = length / (n - 1)
+    kx = torch.fft.fftfreq(n, d=dx, device=device, dtype=torch.float32) * (2 * math.pi)
+    kz = torch.fft.rfftfreq(n, d=dx, device=device, dtype=torch.float32) * (2 * math.pi)
+    k = torch.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    kmin, kmax = 2 * math.pi / length, 2 * math.pi / l_min
+    amp = torch.where((k >= kmin) & (k <= kmax), torch.clamp(k, min=kmin) ** (-11.0 / 6.0),
+                      torch.zeros_like(k))
+    if len(_AMP) >= _AMP_MAX:
+        torch.cuda.synchronize(device)   # no queued kernel still reads the evicted spectrum
+        _AMP.pop(next(iter(_AMP)))
+    built = torch.cuda.Event()
+    built.record(use)
+    _AMP[key] = (amp, built)
+    return amp
+
+
 def field_on_device(spec: FieldSpec, device) -> torch.Tensor:
     """(n, n, n) float32 electron density [m^-3] built on the device."""
     n = spec.side
@@ -83,20 +114,12 @@ def field_on_device(spec: FieldSpec, device) -> torch.Tensor:
     noise = torch.randn((n, n, n), generator=gen, device=device, dtype=torch.float32)
     F = torch.fft.rfftn(noise)
     del noise
-    dx = spec.length / (n - 1)
-    kx = torch.fft.fftfreq(n, d=dx, device=device, dtype=torch.float32) * (2 * math.pi)
-    kz = torch.fft.rfftfreq(n, d=dx, device=device, dtype=torch.float32) * (2 * math.pi)
-    k = torch.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2)
-    kmin, kmax = 2 * math.pi / spec.length, 2 * math.pi / spec.l_min
-    amp = torch.where((k >= kmin) & (k <= kmax), torch.clamp(k, min=kmin) ** (-11.0 / 6.0),
-                      torch.zeros_like(k))
-    del k
-    F *= amp
-    del amp
+    F *= _spectrum_amp(n, spec.length, spec.l_min, device)
     g = torch.fft.irfftn(F, s=(n, n, n))
     del F
-    g = (g - g.mean()) / g.std()
-    return torch.clamp(spec.n0 * (1.0 + spec.frac * g), min=0.0).to(torch.float32).contiguous()
+    # n0 (1 + frac (g - mean) / std), clipped at 0: one std/mean pass, then in place
+    std, mean = torch.std_mean(g)
+    return g.sub_(mean).mul_(spec.frac * spec.n0 / std).add_(spec.n0).clamp_(min=0.0)
 
 
 def beam_on_device(spec: BeamSpec, length: float, device) -> torch.Tensor:

@@ -166,3 +166,37 @@ def test_advance_queues_drain_and_reset():
     _close(out_state, ref.states[0])
     _close(out_path, ref.path)
     _close(out_tau, ref.tau[0])
+
+
+def test_request_field_statistics_and_shared_spectrum():
+    """The serving Kolmogorov field: bit-reproducible per seed, also when the shared spectrum
+    was built on another stream; mean n0, standard deviation frac * n0; no power outside the
+    band 2 pi / length <= |k| <= 2 pi / l_min."""
+    import torch
+
+    from sloth_amd.serving.requests import FieldSpec, field_on_device
+
+    dev = torch.device("cuda", 0)
+    n, L, lmin = 48, 6e-3, 0.5e-3
+    spec = FieldSpec(kind="kolmogorov", side=n, length=L, seed=3, n0=1e24, frac=0.05,
+                     l_min=lmin)
+    side = torch.cuda.Stream(dev)
+    with torch.cuda.stream(side):
+        a = field_on_device(spec, dev)          # builds the spectrum on the side stream
+    b = field_on_device(spec, dev)              # reuses it on the default stream
+    c = field_on_device(FieldSpec(kind="kolmogorov", side=n, length=L, seed=4, n0=1e24,
+                                  frac=0.05, l_min=lmin), dev)
+    torch.cuda.synchronize()
+    assert b.dtype == torch.float32 and b.shape == (n, n, n) and b.is_contiguous()
+    assert torch.equal(a, b) and not torch.equal(b, c)
+    ne = b.double().cpu().numpy()
+    assert abs(ne.mean() / 1e24 - 1.0) < 1e-6
+    assert abs(ne.std() / 0.05e24 - 1.0) < 1e-4
+    F = np.abs(np.fft.rfftn(ne - ne.mean()))
+    dx = L / (n - 1)
+    kx = 2 * np.pi * np.fft.fftfreq(n, d=dx)
+    kz = 2 * np.pi * np.fft.rfftfreq(n, d=dx)
+    k = np.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    outside = (k < 0.95 * 2 * np.pi / L) | (k > 1.05 * 2 * np.pi / lmin)
+    outside[0, 0, 0] = False
+    assert F[outside].max() <= 1e-5 * F.max()
