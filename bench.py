@@ -241,7 +241,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the traced path runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+              "(runs on several GPUs are not launched)", file=sys.stderr)
         sys.exit(2)
 
     import torch

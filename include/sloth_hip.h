@@ -13,7 +13,7 @@
  *
  * The reference is pure Python/JAX: there is no FFI in it.  Each entry point
  * below names the reference function whose work it takes over; the Python
- * mirror in sloth_amd/ binds them with ctypes (INTEGRATION.md shows the stub).
+ * mirror in sloth_amd/ binds them with ctypes (sloth_amd/_native.py).
  */
 #ifndef SLOTH_HIP_H
 #define SLOTH_HIP_H

@@ -1,7 +1,7 @@
 """Shared helpers for GPU-vs-oracle parity tests.
 
 Error measures (the tolerances these are checked against are stated in the
-tests; the north-star bar is 1e-5):
+tests; the parity bar is 1e-5):
   position   ||r_gpu - r_ref||_2 / max(||r_ref||_2, half-extent of the grid)
   direction  ||v_gpu - v_ref||_2 / ||v_ref||_2
   energy     |exp(-tau_gpu) - exp(-tau_ref)| / exp(-tau_ref)
