@@ -127,7 +127,8 @@ int64_t slt_grid_bytes(const slt_grid_desc* g) {
 int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int32_t* keys,
                     slt_stream_t stream) {
     clear_error();
-    if (n < 0 || !g || (n > 0 && (!s0 || !keys))) {
+    if (n < 0 || !g || g->probing_axis < 0 || g->probing_axis > 2 ||
+        (n > 0 && (!s0 || !keys))) {
         set_error("slt_morton_keys: invalid arguments");
         return SLT_ERR_INVALID;
     }
