@@ -146,30 +146,6 @@ __device__ __forceinline__ float4 f4add(float4 a, float4 b) {
     return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 
-// Wavefront sum of v, valid in lane 63: row-wise shifts (DPP row_shr 1, 2, 4, 8)
-// leave each 16-lane row's sum in its last lane, then row_bcast 15 / 31 fold the
-// rows into lane 63.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dppf(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
-                                                                  CTRL, ROW_MASK, 0xf, false));
-}
-
-__device__ __forceinline__ float wave_sum63(float v) {
-    v += dppf<0x111, 0xf>(v);
-    v += dppf<0x112, 0xf>(v);
-    v += dppf<0x114, 0xf>(v);
-    v += dppf<0x118, 0xf>(v);
-    v += dppf<0x142, 0xa>(v);
-    v += dppf<0x143, 0xc>(v);
-    return v;
-}
-
-__device__ __forceinline__ float4 wave_sum63(const float4& v, bool mine) {
-    return make_float4(wave_sum63(mine ? v.x : 0.f), wave_sum63(mine ? v.y : 0.f),
-                       wave_sum63(mine ? v.z : 0.f), wave_sum63(mine ? v.w : 0.f));
-}
-
 __device__ __forceinline__ void f4atomic(float* p, float4 v) {
     atomicAdd(p + 0, v.x);
     atomicAdd(p + 1, v.y);
