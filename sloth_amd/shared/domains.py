@@ -9,7 +9,6 @@ choose it.
 """
 from __future__ import annotations
 
-import numpy as np
 import torch
 
 from .. import _native as _N

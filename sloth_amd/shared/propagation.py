@@ -4,7 +4,6 @@ Mirrors ``shared.propagation.ray_to_Jonesvector`` (src/shared/propagation.py:4-9
 """
 from __future__ import annotations
 
-import numpy as np
 import torch
 
 from .. import _native as N
