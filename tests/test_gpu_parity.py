@@ -8,7 +8,7 @@ and on path length; in-box step counts may differ by one step for at most
 import numpy as np
 import pytest
 
-from _parity import C, assert_parity, errors, gpu_trace, oracle_trace, rays
+from _parity import C, assert_parity, gpu_trace, oracle_trace, rays
 
 pytestmark = pytest.mark.gpu
 
