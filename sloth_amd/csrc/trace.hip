@@ -216,9 +216,10 @@ __device__ __forceinline__ int job_of_unit(const slt_job_table* __restrict__ T, 
 // The table is read through T and its work queues are updated through
 // `queues` (the same allocation, disjoint bytes): with the two kept apart the
 // compiler knows the job entries do not change during the launch and reads
-// them with scalar loads into SGPRs.  Four wavefronts per SIMD, as the offline
-// kernel runs: the few values that do not fit in 128 VGPRs spill only around a
-// unit, not inside the step loop.
+// them with scalar loads into SGPRs.  Four wavefronts per SIMD: the few values
+// that do not fit in 128 VGPRs spill only around a unit, not inside the step
+// loop.  (At five, the offline kernel's occupancy, it spills inside the step
+// loop and the serving mix ran 3-4 % slower.)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 advance_kernel(const slt_job_table* __restrict__ T, uint32_t* __restrict__ queues) {
     const int n_units = T->n_units;

@@ -125,10 +125,13 @@ class ServingEngine:
     ``blocks``: workgroups of the persistent step-batch kernel; 1024 = one resident
     round on MI355X (256 CUs x 4 SIMDs x 4 wavefronts at its register count).
     ``batch_work``: target ray-steps per batch (the batch length in steps is this
-    over the rays resident); smaller batches admit new requests sooner.
+    over the rays resident, at most 4096); smaller batches admit new requests sooner,
+    but each batch costs ~0.35 ms of unit dealing and state traffic whatever its
+    length.  On the serving mix, 4e9 (at most ~30 ms of work) gave higher throughput
+    and lower p50/p99 latency than 5e8-2e9, and longer batches gained no more.
     """
     def __init__(self, *, max_slots: int = 16, hbm_budget: int | None = None,
-                 blocks: int = 1024, batch_work: float = 1e9, use_graph: bool = True,
+                 blocks: int = 1024, batch_work: float = 4e9, use_graph: bool = True,
                  step_fraction: float = 0.5, keep_outputs: bool = True, device=None):
         self.dev = device if device is not None else N.require_device()
         if hbm_budget is None:
