@@ -207,7 +207,8 @@ class ServingEngine:
             t.record_stream(self.main)
         sched = int(nsub.astype(np.int64).sum())
         # steps a ray needs to cross the box along the probing axis; after that the
-        # remaining steps are straight-line moves and go in one batch
+        # remaining steps are straight-line moves and finish in the batch that
+        # reaches this estimate
         ax = mat.grid.axis
         span = float(mat.grid.coords[ax][-1]) - float(mat.grid.coords[ax][0])
         inbox = int(math.ceil(span / (299792458.0 * float(h[0])))) + 4 if h.size and h[0] > 0 else sched
@@ -255,7 +256,11 @@ class ServingEngine:
         table.n_jobs = len(runnable)
         spans = []
         for i, a in enumerate(runnable):
-            end = a.sched if a.cursor >= a.inbox_estimate else min(a.sched, a.cursor + k)
+            end = min(a.sched, a.cursor + k)
+            if end >= a.inbox_estimate:
+                # the rays have crossed the box by the end of this batch: the rest of the
+                # schedule is one straight-line move per ray, so finish it here
+                end = a.sched
             table.jobs[i] = self._job(a, a.cursor, end)
             spans.append(end)
         N.check(N.lib().slt_job_table_prepare(ctypes.byref(table)))
