@@ -87,6 +87,48 @@ def test_trace_rays_that_miss_the_grid():
     np.testing.assert_allclose(r.path.cpu().numpy(), 2 * L, rtol=1e-6)
 
 
+def test_trace_non_finite_rays_leave_the_others_alone():
+    """NaN or infinite start states come out non-finite and do not change any other ray's
+    result (bit for bit).  A non-finite position is outside the box from the start and counts
+    no steps; a non-finite velocity leaves the box after the first step, which counts once
+    when the ray starts on the box face.  As in the reference, where NaN propagates through
+    the interpolation, the detector histogram then drops such rays."""
+    from sloth_amd.core import engine
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import kolmogorov_field
+
+    L = 6e-3
+    ne, coords = kolmogorov_field(40, L, seed=9)
+    grid = engine.PackedGrid(ne, *coords, 2 * np.pi * 299792458.0 / 532e-9, inv_brems=True,
+                             Te=20.0, Z=1.0)
+    good = Beam(3000, 2.5e-3, 0.0, L / 2, beam_type="square", seeded=True).s0
+    bad = good[:, :6].copy()
+    bad[0, 0] = np.nan                    # position
+    bad[4, 1] = np.nan                    # velocity
+    bad[2, 2] = np.inf                    # position
+    bad[5, 3] = np.inf                    # velocity
+    bad[1, 4] = -np.inf
+    bad[0, 5] = 1e38                      # finite but far outside every coordinate range
+    mixed = np.concatenate([good[:, :1500], bad, good[:, 1500:]], axis=1)
+    t = np.array([0.0, 0.6 * L / 299792458.0, 1.5 * L / 299792458.0])
+    a = engine.trace(grid, good, t, save_all=True, want_path=True)
+    b = engine.trace(grid, mixed, t, save_all=True, want_path=True)
+    keep = np.r_[0:1500, 1506:mixed.shape[1]]
+    st_a, st_b = a.states.cpu().numpy(), b.states.cpu().numpy()
+    np.testing.assert_array_equal(st_b[:, :, keep], st_a)
+    np.testing.assert_array_equal(b.tau.cpu().numpy()[:, keep], a.tau.cpu().numpy())
+    np.testing.assert_array_equal(b.path.cpu().numpy()[keep], a.path.cpu().numpy())
+    np.testing.assert_array_equal(b.steps.cpu().numpy()[keep], a.steps.cpu().numpy())
+    bad_steps = b.steps.cpu().numpy()[1500:1506].tolist()
+    assert bad_steps[0] == bad_steps[2] == bad_steps[4] == bad_steps[5] == 0, bad_steps
+    assert bad_steps[1] <= 1 and bad_steps[3] <= 1, bad_steps
+    assert int(b.total_steps.item()) == int(a.total_steps.item()) + sum(bad_steps)
+    fin = st_b[-1][:, 1500:1506]
+    assert not np.isfinite(fin[:, :5]).all(axis=0).any()
+    assert np.isfinite(fin[:, 5]).all()   # the far-away ray stays finite and moves straight
+    np.testing.assert_allclose(fin[0, 5], 1e38, rtol=1e-6)
+
+
 def test_const_device_shares_reads_across_streams_and_evicts():
     """Small read-only uploads are shared by content; a hit on another stream waits for the
     upload; entries past the cap are evicted and new ones still hold the right values."""
