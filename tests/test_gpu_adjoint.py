@@ -127,7 +127,7 @@ def test_adjoint_matches_oracle_central_fd(orc, cl):
     grad = ne_t.grad.double().cpu().numpy()
     L0 = _oracle_loss(orc, ne.astype(np.float64), c64, s64, nsub, h32, omega, tp, te_, cl,
                       pos_scale)
-    assert float(loss) == pytest.approx(L0, rel=1e-4)
+    assert float(loss.detach()) == pytest.approx(L0, rel=1e-4)
 
     rng = np.random.default_rng(5)
     cand = np.argwhere(np.abs(grad) > 0.05 * np.abs(grad).max())
