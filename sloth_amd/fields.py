@@ -78,7 +78,7 @@ def gaussian_random_field(n, length, *, l_max, l_min, exponent=-11.0 / 3.0, seed
     band = (k >= kmin) & (k <= kmax)
     amp = np.zeros_like(k)
     amp[band] = k[band] ** (exponent / 2.0)
-    g = np.fft.irfftn(F * amp, s=(n, n, n))
+    g = np.fft.irfftn(F * amp, s=(n, n, n), axes=(0, 1, 2))
     g = g - g.mean()
     g = g / (g.std() if normalise == "std" else np.abs(g).max())
     return g.astype(np.float32)

@@ -143,6 +143,8 @@ class ServingEngine:
             raise ValueError("blocks must be a positive multiple of 8")
         self.blocks = int(blocks)
         self.batch_work = float(batch_work)
+        # rays that give every wavefront of the persistent launch two 64-ray units
+        self.fill_rays = 2 * 64 * (self.blocks * 4)
         self.step_fraction = step_fraction
         self.keep_outputs = keep_outputs
         self.main = torch.cuda.Stream(self.dev)
@@ -281,6 +283,19 @@ class ServingEngine:
                     a.finished = True
                     self._finalize(a)
 
+    def _worth_launching(self, runnable, active, pending, inflight) -> bool:
+        """Whether to queue a batch over ``runnable`` now.  With no batch left on the GPU,
+        always.  Behind a running batch, only a batch that gives every resident wavefront
+        work: a smaller one (requests trickling out of setup one at a time) waits, so the
+        requests still in setup can join it instead of each running in a batch that leaves
+        most of the GPU idle while it drains."""
+        if not any(not e.query() for e in inflight):
+            return True
+        if not pending and all(a.finished or a.ready.query() for a in active):
+            return True
+        rays = sum(a.mat.rays.shape[1] for a in runnable)
+        return rays >= self.fill_rays
+
     # -- main loop ------------------------------------------------------------
     def run(self, requests, *, realtime: bool = True, max_lookahead: int = 2):
         """Serve ``requests`` (arrival times in seconds from the start).
@@ -313,7 +328,7 @@ class ServingEngine:
             progressed = False
             runnable = [a for a in active if not a.finished and a.ready.query()]
             runnable = runnable[:N.SLT_MAX_JOBS]
-            if runnable:
+            if runnable and self._worth_launching(runnable, active, pending, inflight):
                 self._launch_batch(runnable, inflight)
                 progressed = True
             # One setup per pass, after the batch is queued: the GPU integrates

@@ -111,15 +111,33 @@ def field_on_device(spec: FieldSpec, device) -> torch.Tensor:
         raise ValueError(f"unknown field kind {spec.kind!r}")
     gen = torch.Generator(device=device)
     gen.manual_seed(int(spec.seed))
-    noise = torch.randn((n, n, n), generator=gen, device=device, dtype=torch.float32)
-    F = torch.fft.rfftn(noise)
-    del noise
+    F = _half_spectrum_noise(n, gen, device)
     F *= _spectrum_amp(n, spec.length, spec.l_min, device)
     g = torch.fft.irfftn(F, s=(n, n, n))
     del F
-    # n0 (1 + frac (g - mean) / std), clipped at 0: one std/mean pass, then in place
+    # n0 (1 + frac (g - mean) / std), clipped at 0: one std/mean pass, then
+    # g * a + b in place with a = frac n0 / std, b = n0 - mean a
     std, mean = torch.std_mean(g)
-    return g.sub_(mean).mul_(spec.frac * spec.n0 / std).add_(spec.n0).clamp_(min=0.0)
+    a = (spec.frac * spec.n0) / std
+    return g.mul_(a).add_(spec.n0 - mean * a).clamp_(min=0.0)
+
+
+def _half_spectrum_noise(n: int, gen: torch.Generator, device) -> torch.Tensor:
+    """White complex Gaussian noise on the (n, n, n / 2 + 1) half spectrum of a real n^3
+    field, drawn directly in k-space (the spectrum of real white noise, without the noise
+    field and its forward transform).  In the planes kz = 0 and, for even n, kz = n / 2 a
+    mode's conjugate partner (-kx, -ky) lies in the same plane, so those planes are made
+    Hermitian, P <- (P + conj(P(-kx, -ky))) / sqrt(2), which keeps every mode's expected
+    power and makes the self-conjugate modes real; the real inverse transform is then the
+    exact one."""
+    F = torch.view_as_complex(
+        torch.randn((n, n, n // 2 + 1, 2), generator=gen, device=device, dtype=torch.float32))
+    neg = engine.const_device((-np.arange(n, dtype=np.int64)) % n, device)  # -k mod n
+    for kz in ((0, n // 2) if n % 2 == 0 else (0,)):
+        P = F[:, :, kz]
+        Q = P.index_select(0, neg).index_select(1, neg).conj()
+        P.add_(Q).mul_(1.0 / math.sqrt(2.0))
+    return F
 
 
 def beam_on_device(spec: BeamSpec, length: float, device) -> torch.Tensor:
