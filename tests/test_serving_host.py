@@ -96,3 +96,39 @@ def test_job_table_prepare_units_and_errors():
     assert L.slt_job_table_prepare(ctypes.byref(t)) == N.SLT_ERR_INVALID
     with pytest.raises(ValueError):
         N.check(N.SLT_ERR_INVALID)
+
+
+def test_batch_launch_policy():
+    """A batch goes out at once when the GPU has none left or nothing is still in setup;
+    behind a running batch only when it fills every resident wavefront."""
+    from types import SimpleNamespace as NS
+
+    from sloth_amd.serving.engine import ServingEngine
+
+    class Ev:
+        def __init__(self, done):
+            self.done = done
+
+        def query(self):
+            return self.done
+
+    def act(rays, ready=True, finished=False):
+        return NS(mat=NS(rays=np.zeros((6, rays), np.float32)), ready=Ev(ready),
+                  finished=finished)
+
+    eng = NS(fill_rays=2 * 64 * 1024 * 4)
+    go = lambda run, active, pending, inflight: ServingEngine._worth_launching(  # noqa: E731
+        eng, run, active, pending, inflight)
+    small, big = act(10_000), act(600_000)
+    setting_up = act(50_000, ready=False)
+    # nothing on the GPU: always
+    assert go([small], [small, setting_up], deque(), deque())
+    assert go([small], [small], deque([_req(9)]), deque([Ev(True)]))
+    # a batch still running, requests still in setup: only a batch that fills the GPU
+    running = deque([Ev(True), Ev(False)])
+    assert not go([small], [small, setting_up], deque(), running)
+    assert not go([small], [small], deque([_req(9)]), running)
+    assert go([big], [big, setting_up], deque(), running)
+    assert go([small, big], [small, big, setting_up], deque(), running)
+    # a batch still running, but every request is set up: no reason to wait
+    assert go([small], [small, act(5, finished=True)], deque(), running)
