@@ -97,8 +97,11 @@ struct AdjArgs {
     int nblocks;
 };
 
+// Five wavefronts per SIMD, as the offline trace kernel: at 96 VGPRs the
+// values that spill are touched around the save points and in the rare
+// no-stage-in-the-box branch, not in the step loop (7.9 vs 8.3 ms at four).
 template <bool KAPPA>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 forward_ckpt_kernel(AdjArgs A) {
     const int lb = xcd_block(blockIdx.x, gridDim.x, A.nblocks);
     if (lb < 0) return;
