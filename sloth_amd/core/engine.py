@@ -267,10 +267,13 @@ class TraceResult:
 def ray_order(grid: PackedGrid, s0: torch.Tensor, stream=None) -> torch.Tensor:
     """Z-order of the rays' transverse start positions (int32 permutation)."""
     n = s0.shape[1]
-    keys = torch.empty(n, dtype=torch.int32, device=s0.device)
-    N.check(N.lib().slt_morton_keys(s0.data_ptr(), n, ctypes.byref(grid.desc), keys.data_ptr(),
-                                    N.stream_handle(stream)))
-    return torch.sort(keys, stable=True).indices.to(torch.int32)
+    # the keys, the kernel that writes them and the sort that reads them all go
+    # on `stream`, so the sort cannot run ahead of the keys
+    with torch.cuda.stream(stream):
+        keys = torch.empty(n, dtype=torch.int32, device=s0.device)
+        N.check(N.lib().slt_morton_keys(s0.data_ptr(), n, ctypes.byref(grid.desc),
+                                        keys.data_ptr(), N.stream_handle(stream)))
+        return torch.sort(keys, stable=True).indices.to(torch.int32)
 
 
 def as_rays(s0, device) -> torch.Tensor:
