@@ -357,6 +357,25 @@ int slt_adjoint(const void* packed, const slt_grid_desc* g, const slt_adjoint_ar
                 slt_stream_t stream);
 
 /*
+ * slt_adjoint without the tape argument, allocating nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The reverse pass's tape lives in a library-owned device array
+ * of SLT_ADJ_SCRATCH_BYTES, one per GPU, held from its first use until the
+ * process ends; a->tape must be NULL.  Limit: slt_tape_bytes(n_rays,
+ * ckpt_every) <= SLT_ADJ_SCRATCH_BYTES, i.e. n_rays * ckpt_every <= 4793490
+ * (e.g. 299593 rays at K = 16); larger calls return SLT_ERR_UNSUPPORTED
+ * before any device work.  The checkpoints, seeds and grid_adj stay
+ * caller-owned, as in slt_adjoint, and the result is the same.  Every call on
+ * one GPU shares that GPU's array, so calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller.  The
+ * stream must belong to the current device.
+ */
+#define SLT_ADJ_SCRATCH_BYTES ((int64_t)128 << 20)
+int slt_adjoint_noalloc(const void* packed, const slt_grid_desc* g, const slt_adjoint_args* a,
+                        slt_stream_t stream);
+
+/*
  * d loss / d n_e from the voxel adjoint: the transposed numpy.gradient stencil
  * (src/core/propagator.py:90-92) times grad_coeff, plus d kappa / d n_e
  * (propagator.py:147-164) times the kappa adjoint.  Arguments as in

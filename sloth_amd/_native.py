@@ -22,10 +22,11 @@ EXPORTED = (
     "slt_kappa", "slt_trace", "slt_jones", "slt_histogram2d", "slt_morton_keys",
     "slt_ray_state_bytes", "slt_ray_state_init", "slt_job_table_prepare", "slt_advance",
     "slt_checkpoint_bytes", "slt_tape_bytes", "slt_forward_ckpt", "slt_adjoint",
-    "slt_adjoint_unpack", "slt_rtm", "slt_beam",
+    "slt_adjoint_unpack", "slt_rtm", "slt_beam", "slt_adjoint_noalloc",
 )
 
 SLT_MAX_JOBS = 64
+SLT_ADJ_SCRATCH_BYTES = 128 << 20
 SLT_WORK_QUEUES = 8
 SLT_MAX_OPTICS = 32
 SLT_OPTIC_TRAVEL = 0
@@ -174,6 +175,8 @@ def lib():
     L.slt_tape_bytes.restype = i64
     L.slt_forward_ckpt.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(AdjointArgs), vp]
     L.slt_adjoint.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(AdjointArgs), vp]
+    L.slt_adjoint_noalloc.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(AdjointArgs),
+                                      vp]
     L.slt_adjoint_unpack.argtypes = [vp, ctypes.POINTER(GridDesc), vp, vp, vp, vp, dbl, vp, i64,
                                      vp, i64, dbl, dbl, dbl, dbl, dbl, vp, vp]
     for name in EXPORTED:

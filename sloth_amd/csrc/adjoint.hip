@@ -19,6 +19,8 @@
 //    plus d kappa / d n_e times the kappa adjoint.
 #include <math.h>
 
+#include <mutex>
+
 #include "integrate.h"
 
 namespace slt {
@@ -666,6 +668,91 @@ extern "C" int slt_adjoint(const void* packed, const slt_grid_desc* g, const slt
         hipLaunchKernelGGL(adjoint_kernel<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream,
                            A);
     return check_launch("slt_adjoint");
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free reverse pass: the tape in a library-owned device array.
+// ---------------------------------------------------------------------------
+
+// The reverse pass's tape for slt_adjoint_noalloc, one copy per GPU (held from
+// the library's load on that GPU until the process ends).  In host code this
+// name is a stand-in for the symbol, not a device address: the address of each
+// GPU's copy comes from hipGetSymbolAddress and is what the kernel receives.
+static __device__ __align__(256) unsigned char scratch[SLT_ADJ_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of the current GPU's scratch, looked up once per device.
+int scratch_address(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error("slt_adjoint_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+        return SLT_ERR_HIP;
+    }
+    if (dev < 0 || dev >= kMaxScratchDevices) {
+        set_error("slt_adjoint_noalloc: device %d beyond the %d supported", dev,
+                  kMaxScratchDevices);
+        return SLT_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("slt_adjoint_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return SLT_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SLT_OK;
+}
+
+}  // namespace
+
+extern "C" int slt_adjoint_noalloc(const void* packed, const slt_grid_desc* g,
+                                   const slt_adjoint_args* a, slt_stream_t stream) {
+    clear_error();
+    AdjArgs A;
+    int rc = fill_adj_args(packed, g, a, &A, "slt_adjoint_noalloc");
+    if (rc) return rc;
+    if (a->tape) {
+        set_error("slt_adjoint_noalloc: tape must be NULL (the tape is the library's scratch)");
+        return SLT_ERR_INVALID;
+    }
+    if (A.n == 0) return SLT_OK;
+    if (!A.seed_state || !A.grid_adj) {
+        set_error("slt_adjoint_noalloc: seed_state and grid_adj are required");
+        return SLT_ERR_INVALID;
+    }
+    // the tape holds 28 bytes per ray and step of an interval: n * K <= cap, tested
+    // without forming the product (it could overflow)
+    const int64_t cap = SLT_ADJ_SCRATCH_BYTES / 28;
+    if (A.n > cap / A.K) {
+        set_error("slt_adjoint_noalloc: %lld rays at ckpt_every %d exceed the %lld-byte tape "
+                  "scratch (n_rays * ckpt_every <= %lld)",
+                  (long long)A.n, A.K, (long long)SLT_ADJ_SCRATCH_BYTES, (long long)cap);
+        return SLT_ERR_UNSUPPORTED;
+    }
+    void* tape = nullptr;
+    rc = scratch_address(&tape);
+    if (rc) return rc;
+    // every tape slot is written by the replay before the reverse sweep reads it,
+    // so the scratch needs no clearing between calls
+    A.tape = static_cast<float*>(tape);
+    const int grid = xcd_pad(A.nblocks);
+    if (g->nchan >= 4)
+        hipLaunchKernelGGL(adjoint_kernel<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, A);
+    else
+        hipLaunchKernelGGL(adjoint_kernel<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream,
+                           A);
+    return check_launch("slt_adjoint_noalloc");
 }
 
 extern "C" int slt_adjoint_unpack(const float* grid_adj, const slt_grid_desc* g, const float* ne,

@@ -218,6 +218,23 @@ def test_adjoint_rejections(L):
         _expect(L, L.slt_adjoint(FAKE, ctypes.byref(g), ctypes.byref(aargs(**{missing: None})),
                                  None), N.SLT_ERR_INVALID, "slt_adjoint")
 
+    # the allocation-free reverse pass: its tape is the library's, so a caller's
+    # tape and a tape beyond the scratch are refused before any device work
+    noalloc = L.slt_adjoint_noalloc
+    _expect(L, noalloc(FAKE, ctypes.byref(g), ctypes.byref(aargs()), None), N.SLT_ERR_INVALID,
+            "tape must be NULL")
+    for missing in ("seed_state", "grid_adj"):
+        _expect(L, noalloc(FAKE, ctypes.byref(g),
+                           ctypes.byref(aargs(tape=None, **{missing: None})), None),
+                N.SLT_ERR_INVALID, "slt_adjoint_noalloc")
+    cap = N.SLT_ADJ_SCRATCH_BYTES // 28
+    for n, K in ((cap // 16 + 1, 16), (cap + 1, 1), (2, cap), (2 ** 37, 2 ** 30)):
+        _expect(L, noalloc(FAKE, ctypes.byref(g),
+                           ctypes.byref(aargs(tape=None, n_rays=n, ckpt_every=K)), None),
+                N.SLT_ERR_UNSUPPORTED, "scratch")
+    _expect(L, noalloc(FAKE, ctypes.byref(g), ctypes.byref(aargs(tape=None, n_rays=0)), None),
+            N.SLT_OK, None)
+
     def unpack(adj=FAKE, te=None, zz=None, n_te=0, g=g):
         return L.slt_adjoint_unpack(adj, ctypes.byref(g), FAKE, FAKE, FAKE, FAKE, -1.0, te, n_te,
                                     zz, 1, 1e15, 1.6e-19, 8.85e-12, 3e8, 0.0, FAKE, None)
