@@ -79,7 +79,7 @@ def load():
     lib.orc_interp_points.argtypes = [i32, i32, i32, P(d), P(d), P(d), P(d), i64, P(d), P(d)]
     lib.orc_trace.argtypes = [i32, i32, i32, P(d), P(d), P(d), P(d), P(d), P(d), P(d),
                               i64, P(d), i32, P(ctypes.c_int32), P(d), i32, P(d),
-                              P(ctypes.c_int64), i32]
+                              P(ctypes.c_int64), P(d), i32]
     lib.orc_max_threads.restype = ctypes.c_int
     _lib = lib
     return lib
@@ -150,8 +150,11 @@ def interp_points(coords, values, pts):
     return out
 
 
-def trace(grads, kap, coords, s0, nsub, h, naive=False, nthreads=0):
-    """Fixed-step RK4 trace.  Returns (saves[(S+1),8,N], steps[N]).
+def trace(grads, kap, coords, s0, nsub, h, naive=False, nthreads=0, face_gap=False):
+    """Fixed-step RK4 trace.  Returns (saves[(S+1),8,N], steps[N]), and with
+    ``face_gap=True`` also gap[N]: per ray, the smallest distance [m] from any stage
+    point of its RK4 steps (the start point excepted) to the surface of the closed
+    grid box.
 
     Rows of each save: x, y, z, vx, vy, vz, tau, path (length of the polyline through
     the step end points).
@@ -167,11 +170,15 @@ def trace(grads, kap, coords, s0, nsub, h, naive=False, nthreads=0):
     nseg = nsub.size
     saves = np.empty((nseg + 1, 8, n))
     steps = np.empty(n, dtype=np.int64)
+    gap = np.empty(n) if face_gap else None
     lib.orc_trace(cx.size, cy.size, cz.size, _dp(cx), _dp(cy), _dp(cz),
                   _dp(dndx), _dp(dndy), _dp(dndz), _dp(kp) if kp is not None else None,
                   n, _dp(s0), nseg, nsub.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                   _dp(h), 1 if naive else 0, _dp(saves),
-                  steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(nthreads))
+                  steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                  _dp(gap) if gap is not None else None, int(nthreads))
+    if face_gap:
+        return saves, steps, gap
     return saves, steps
 
 

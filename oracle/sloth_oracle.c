@@ -189,6 +189,25 @@ static int orc_segment_touches_box(const orc_grid* G, const double p[3], const d
     return s0 <= s1 * (1.0 + 1e-9) + 1e-30;
 }
 
+/* Distance from p to the surface of the closed box (inside: to the nearest face;  */
+/* outside: Euclidean distance to the box).  A stage point closer than the gap    */
+/* between two trajectories can be inside for one and outside for the other.     */
+static double orc_face_gap(const orc_grid* G, const double p[3]) {
+    double out2 = 0.0, in = INFINITY;
+    int outside = 0;
+    for (int a = 0; a < 3; ++a) {
+        const double lo = G->coord[a][0], hi = G->coord[a][G->n[a] - 1];
+        double e = 0.0;
+        if (p[a] < lo) e = lo - p[a];
+        if (p[a] > hi) e = p[a] - hi;
+        if (e > 0.0) outside = 1;
+        out2 += e * e;
+        const double d = fmin(p[a] - lo, hi - p[a]);
+        if (d < in) in = d;
+    }
+    return outside ? sqrt(out2) : in;
+}
+
 static void orc_rhs(const orc_grid* G, const double r[3], double a[3], double* kap, int* inside) {
     double o[4];
     int in = orc_interp4(G, r, o);
@@ -207,11 +226,14 @@ static void orc_rhs(const orc_grid* G, const double r[3], double a[3], double* k
 /*           stage points lies inside the closed grid box)                 */
 /*   naive   1: every step is a full RK4 step; 0: steps whose straight     */
 /*           segment cannot touch the box are taken as exact linear moves  */
+/*   face_gap  per ray (or NULL): the smallest orc_face_gap of any stage   */
+/*           point of its RK4 steps, except the start point itself (which  */
+/*           both a checked trace and this one take unrounded)             */
 /* ---------------------------------------------------------------------- */
 void orc_trace(int nx, int ny, int nz, const double* cx, const double* cy, const double* cz,
                const double* dndx, const double* dndy, const double* dndz, const double* kappa,
                int64_t n, const double* s0, int nseg, const int32_t* nsub, const double* h,
-               int naive, double* saves, int64_t* steps, int nthreads) {
+               int naive, double* saves, int64_t* steps, double* face_gap, int nthreads) {
     orc_grid G;
     G.n[0] = nx; G.n[1] = ny; G.n[2] = nz;
     G.coord[0] = cx; G.coord[1] = cy; G.coord[2] = cz;
@@ -224,8 +246,9 @@ void orc_trace(int nx, int ny, int nz, const double* cx, const double* cy, const
     const int64_t plane = 8 * n;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t ray = 0; ray < n; ++ray) {
-        double r[3], v[3], tau = 0.0, path = 0.0;
+        double r[3], v[3], tau = 0.0, path = 0.0, gap = INFINITY;
         int64_t nstep = 0;
+        int moved = 0;
         for (int c = 0; c < 3; ++c) { r[c] = s0[c * n + ray]; v[c] = s0[(3 + c) * n + ray]; }
 #define ORC_SAVE(sidx)                                                      \
         do {                                                                \
@@ -241,6 +264,7 @@ void orc_trace(int nx, int ny, int nz, const double* cx, const double* cy, const
                     const double sp = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
                     for (int c = 0; c < 3; ++c) r[c] += hh * v[c];
                     path += hh * sp;
+                    moved = 1;
                     continue;
                 }
                 int inside = 0;
@@ -253,6 +277,13 @@ void orc_trace(int nx, int ny, int nz, const double* cx, const double* cy, const
                 orc_rhs(&G, r3, a3, &k3, &inside);
                 for (int c = 0; c < 3; ++c) { r4[c] = r[c] + hh * v3[c]; v4[c] = v[c] + hh * a3[c]; }
                 orc_rhs(&G, r4, a4, &k4, &inside);
+                if (face_gap) {
+                    if (moved) gap = fmin(gap, orc_face_gap(&G, r));
+                    gap = fmin(gap, orc_face_gap(&G, r2));
+                    gap = fmin(gap, orc_face_gap(&G, r3));
+                    gap = fmin(gap, orc_face_gap(&G, r4));
+                }
+                moved = 1;
                 double dr2 = 0.0;
                 for (int c = 0; c < 3; ++c) {
                     const double dr = hh / 6.0 * (v[c] + 2.0 * v2[c] + 2.0 * v3[c] + v4[c]);
@@ -268,6 +299,7 @@ void orc_trace(int nx, int ny, int nz, const double* cx, const double* cy, const
         }
 #undef ORC_SAVE
         if (steps) steps[ray] = nstep;
+        if (face_gap) face_gap[ray] = gap;
     }
 }
 

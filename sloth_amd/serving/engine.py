@@ -84,6 +84,9 @@ class Response:
     amp: torch.Tensor | None = None  # (N,) float64 exp(-tau)
     image: torch.Tensor | None = None  # (bins, bins) counts or exp(-tau)-weighted sums
     path: torch.Tensor | None = None
+    state: torch.Tensor | None = None  # (6, N) float32 final x, y, z, vx, vy, vz
+    tau: torch.Tensor | None = None    # (N,) float64 final optical depth (absorption only)
+    steps: torch.Tensor | None = None  # (N,) int32 in-box RK4 steps per ray
     ray_steps: int = 0
     arrival: float = 0.0
     completed: float = math.nan     # seconds since the run started (GPU clock)
@@ -107,6 +110,7 @@ class _Active:
     out_state: torch.Tensor
     out_tau: torch.Tensor
     out_path: torch.Tensor
+    out_steps: torch.Tensor
     counter: torch.Tensor
     ready: torch.cuda.Event
     cursor: int = 0
@@ -201,11 +205,12 @@ class ServingEngine:
             out_state = torch.empty((6, n), dtype=torch.float32, device=self.dev)
             out_tau = torch.zeros(n, dtype=torch.float64, device=self.dev)
             out_path = torch.empty(n, dtype=torch.float64, device=self.dev)
+            out_steps = torch.empty(n, dtype=torch.int32, device=self.dev)
             counter = torch.zeros(1, dtype=torch.int64, device=self.dev)
             ready = torch.cuda.Event()
             ready.record(self.setup_stream)
         for t in (mat.grid.voxels, state, order, seg_nsub, seg_h, out_state, out_tau, out_path,
-                  counter):
+                  out_steps, counter):
             t.record_stream(self.main)
         sched = int(nsub.astype(np.int64).sum())
         # steps a ray needs to cross the box along the probing axis; after that the
@@ -215,7 +220,7 @@ class ServingEngine:
         span = float(mat.grid.coords[ax][-1]) - float(mat.grid.coords[ax][0])
         inbox = int(math.ceil(span / (299792458.0 * float(h[0])))) + 4 if h.size and h[0] > 0 else sched
         return _Active(req, mat, order, state, seg_nsub, seg_h, sched, min(inbox, sched),
-                       out_state, out_tau, out_path, counter, ready)
+                       out_state, out_tau, out_path, out_steps, counter, ready)
 
     def _job(self, a: _Active, begin: int, end: int) -> N.Job:
         j = N.Job()
@@ -233,7 +238,7 @@ class ServingEngine:
         j.out_state = a.out_state.data_ptr()
         j.out_tau = _f(a.out_tau) if a.mat.grid.inv_brems else None
         j.out_path = a.out_path.data_ptr()
-        j.out_steps = None
+        j.out_steps = a.out_steps.data_ptr()
         j.step_counter = a.counter.data_ptr()
         return j
 
@@ -246,7 +251,10 @@ class ServingEngine:
         a.result = {"rf": rf if self.keep_outputs else None,
                     "amp": amp if self.keep_outputs else None,
                     "image": wsum if wsum is not None else counts,
-                    "path": a.out_path if self.keep_outputs else None}
+                    "path": a.out_path if self.keep_outputs else None,
+                    "state": a.out_state if self.keep_outputs else None,
+                    "tau": a.out_tau if self.keep_outputs and inv else None,
+                    "steps": a.out_steps if self.keep_outputs else None}
         a.done = torch.cuda.Event(enable_timing=True)
         a.done.record(self.main)
 
@@ -347,7 +355,8 @@ class ServingEngine:
                 r = a.req
                 responses[r.rid] = Response(
                     r.rid, "ok", rf=a.result["rf"], amp=a.result["amp"],
-                    image=a.result["image"], path=a.result["path"], ray_steps=steps,
+                    image=a.result["image"], path=a.result["path"], state=a.result["state"],
+                    tau=a.result["tau"], steps=a.result["steps"], ray_steps=steps,
                     arrival=r.arrival, completed=start_ev.elapsed_time(a.done) / 1e3,
                     n_rays=a.mat.rays.shape[1])
                 active.remove(a)
@@ -398,7 +407,8 @@ def run_offline(requests, *, step_fraction: float = 0.5, device=None):
         counts, wsum = histogram2d_device(rf[0], rf[2], mat.edges, mat.edges, weights=amp,
                                           order=res.order)
         out[req.rid] = {"rf": rf, "amp": amp, "image": wsum if wsum is not None else counts,
-                        "path": res.path, "steps": res.total_steps, "state": res.states[0]}
+                        "path": res.path, "steps": res.total_steps, "state": res.states[0],
+                        "nsub": res.nsub, "h": res.h}
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - t0
     for v in out.values():

@@ -50,8 +50,8 @@ def oracle_trace(orc, ne, coords, s0, nsub, h, *, lwl=532e-9, inv_brems=False, T
     # the kernel steps with the fp32-rounded step lengths; give the oracle the same
     h32 = np.asarray(h, dtype=np.float32).astype(np.float64)
     s64 = np.asarray(s0, dtype=np.float32).astype(np.float64)
-    saves, steps = orc.trace(grads, kap, c64, s64, nsub, h32)
-    return {"saves": saves, "steps": steps, "grads": grads, "kappa": kap}
+    saves, steps, gap = orc.trace(grads, kap, c64, s64, nsub, h32, face_gap=True)
+    return {"saves": saves, "steps": steps, "face_gap": gap, "grads": grads, "kappa": kap}
 
 
 def errors(gpu, ref, coords):
@@ -80,17 +80,27 @@ def assert_parity(gpu, ref, coords, tol=1e-5, step_mismatch_frac=1e-3):
     assert np.max(e["dir"]) <= tol, msg
     if "energy" in e:
         # A stage point within the two trajectories' mutual distance (~1e-6 of the extent)
-        # of the box's exit face is inside for one and outside for the other, which moves
+        # of a face of the box is inside for one and outside for the other, which moves
         # that stage's share of tau: the same class of ray as a detector count that moves
-        # across a bin edge.  Such rays may exceed the bar, at most step_mismatch_frac of
-        # them, each by no more than one step's share of its tau.
+        # across a bin edge.  A ray may exceed the bar only if it is shown to be such a
+        # ray -- the oracle trajectory has a stage point within 1e-5 of the extent of a
+        # face (ref["face_gap"]), or the two step counts differ by one -- and then by no
+        # more than one step's share of its tau; at most step_mismatch_frac of the rays.
         en = np.max(e["energy"], axis=0)
         bad = en > tol
         if bad.any():
+            extent = max(float(c[-1]) - float(c[0]) for c in coords)
+            gap = ref.get("face_gap")
+            near = gap <= 1e-5 * extent if gap is not None else np.zeros(bad.shape, bool)
+            explained = near | (np.abs(gpu["steps"] - ref["steps"]) == 1)
+            assert explained[bad].all(), (msg, "rays over the energy bar that neither pass "
+                                          "within 1e-5 of the extent of a face nor differ "
+                                          "by one step", np.flatnonzero(bad & ~explained))
             per_step = gpu["tau"][-1] / np.maximum(gpu["steps"], 1)
             assert bad.mean() <= step_mismatch_frac, (msg, int(bad.sum()))
             assert np.all(en[bad] <= 2.0 * per_step[bad]), (msg, en[bad], per_step[bad])
             msg["energy_rays_over_tol"] = int(bad.sum())
+            msg["energy_rays_near_face"] = int((bad & near).sum())
     assert np.max(e["path"]) <= tol, msg
     ds = np.abs(gpu["steps"] - ref["steps"])
     assert ds.max() <= 1, ("step counts differ by more than one", ds.max())

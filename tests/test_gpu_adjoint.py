@@ -1,5 +1,8 @@
-"""Adjoint (k4) checks: transposed stencil identity, d kappa / d n_e, and dL/dn_e against
-central finite differences of the FP64 oracle's loss on 64 random voxels.
+"""Adjoint (k4) checks: transposed stencil identity, d kappa / d n_e, dL/dn_e against
+central finite differences of the FP64 oracle's loss on 64 voxels drawn from the beam's
+own columns (not from the adjoint's output), the unsorted many-cells-per-flush path
+against the sorted one, and the full config-A problem (256^3, 2e6 rays, K = 16) through
+directional derivatives against central differences of the GPU forward.
 
 The reference has no gradient path; jax.grad through its diffrax solve is not
 runnable here, so the oracle-FD comparison is the pin (the adjoint is that of
@@ -84,6 +87,31 @@ def test_dkappa_dne_matches_fd(orc, ne_val, te, cl):
     assert got == pytest.approx(fd, rel=1e-5)
 
 
+def _crossed_voxels(s0, coords, n_pick, seed):
+    """n_pick distinct voxels drawn uniformly over the (ray, plane) pairs of the beam: ray
+    r's column at its start, offset transversally by 0 or 1 cell (the cell's corners) or,
+    on one axis only, by -1 or 2 (the gradient stencil of a corner reaches that far), at
+    a random plane along the probing axis (z).  The rays are nearly straight, so such a
+    voxel's density moves the loss; the draw does not look at the adjoint's output, so a
+    dropped contribution shows up as a mismatch."""
+    rng = np.random.default_rng(seed)
+    lo = [float(c[0]) for c in coords]
+    d = [(float(c[-1]) - float(c[0])) / (len(c) - 1) for c in coords]
+    n = [len(c) for c in coords]
+    picked = set()
+    while len(picked) < n_pick:
+        r = rng.integers(s0.shape[1])
+        di, dj = (int(v) for v in rng.integers(-1, 3, size=2))
+        if di not in (0, 1) and dj not in (0, 1):
+            continue
+        i = int((float(s0[0, r]) - lo[0]) / d[0]) + di
+        j = int((float(s0[1, r]) - lo[1]) / d[1]) + dj
+        k = int(rng.integers(1, n[2] - 1))
+        if 0 <= i < n[0] and 0 <= j < n[1]:
+            picked.add((i, j, k))
+    return sorted(picked)
+
+
 def _oracle_loss(orc, ne64, c64, s64, nsub, h32, omega, tp, te_, cl, pos_scale):
     grads = orc.gradients(ne64, *c64, omega)
     kap = orc.kappa(ne64, 20.0, 1.0, omega, coulomb_log=cl)
@@ -129,11 +157,9 @@ def test_adjoint_matches_oracle_central_fd(orc, cl):
                       pos_scale)
     assert float(loss.detach()) == pytest.approx(L0, rel=1e-4)
 
-    rng = np.random.default_rng(5)
-    cand = np.argwhere(np.abs(grad) > 0.05 * np.abs(grad).max())
-    pick = cand[rng.choice(len(cand), 64, replace=False)]
+    pick = _crossed_voxels(s0, coords, 64, seed=5)
     fd, ad = [], []
-    for idx in map(tuple, pick):
+    for idx in pick:
         x = float(ne[idx])
         d = 1e-4 * x
         up = ne.astype(np.float64)
@@ -146,7 +172,10 @@ def test_adjoint_matches_oracle_central_fd(orc, cl):
         ad.append(grad[idx])
     fd, ad = np.array(fd), np.array(ad)
     rel = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
-    print("adjoint vs FD: normwise", rel, "worst", np.max(np.abs(ad - fd) / np.abs(fd)))
+    print("adjoint vs FD: normwise", rel, "worst", np.max(np.abs(ad - fd)) / np.abs(fd).max(),
+          "of the largest")
+    # the sample is informative: most drawn voxels move the loss
+    assert np.mean(np.abs(fd) > 1e-3 * np.abs(fd).max()) >= 0.5
     assert rel <= 2e-3
     assert np.all(np.abs(ad - fd) <= 2e-2 * np.abs(fd) + 2e-3 * np.abs(fd).max())
 
@@ -225,11 +254,9 @@ def test_adjoint_full_size_grid_subset_fd(orc):
         return float(np.mean(((fin[0:2] - tp) ** 2).sum(0) / pos_scale ** 2
                              + (np.exp(-fin[6]) - te_) ** 2))
 
-    rng = np.random.default_rng(9)
-    cand = np.argwhere(np.abs(grad) > 0.05 * np.abs(grad).max())
-    pick = cand[rng.choice(len(cand), 64, replace=False)]
+    pick = _crossed_voxels(s0, coords, 64, seed=9)
     fd, ad = [], []
-    for idx in map(tuple, pick):
+    for idx in pick:
         x = ne64[idx]
         d = 1e-4 * x
         vals = []
@@ -243,5 +270,102 @@ def test_adjoint_full_size_grid_subset_fd(orc):
         ad.append(grad[idx])
     fd, ad = np.array(fd), np.array(ad)
     rel = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
-    print("full-size adjoint vs FD: normwise", rel, "worst", np.max(np.abs(ad - fd) / np.abs(fd)))
+    print("full-size adjoint vs FD: normwise", rel, "worst",
+          np.max(np.abs(ad - fd)) / np.abs(fd).max(), "of the largest")
+    assert np.mean(np.abs(fd) > 1e-3 * np.abs(fd).max()) >= 0.5
     assert rel <= 2e-3
+    assert np.all(np.abs(ad - fd) <= 2e-2 * np.abs(fd) + 2e-3 * np.abs(fd).max())
+
+
+def test_adjoint_unsorted_rays_many_cells_per_flush():
+    """Rays in their seeded random order (sort=False): the 64 lanes of a wavefront are
+    scattered over ~290 beam columns of the 24^3 grid, so a collective flush (all lanes
+    cross a cell face along the probing axis together) holds ~57 distinct cells, more
+    than the 16 LDS slots, and the lanes beyond them scatter their own moments.  The
+    gradient must equal the sorted run's (same contributions in another summation order)
+    and the oracle's central differences on crossed voxels."""
+    import torch
+
+    from sloth_amd.adjoint import trace_states
+
+    ne, coords, s0, cl = _setup(rays=3000)
+    grads = []
+    for sort in (True, False):
+        ne_t = torch.from_numpy(ne).cuda().requires_grad_(True)
+        state, tau, _ = trace_states(ne_t, s0, *coords, inv_brems=True, Te=20.0, Z=1.0,
+                                     coulomb_log=cl, sort=sort)
+        (state[0].double().sum() * 1e3 + state[1].double().sum() * 1e3 + tau.sum()).backward()
+        grads.append(ne_t.grad.double().cpu().numpy())
+    rel = np.linalg.norm(grads[1] - grads[0]) / np.linalg.norm(grads[0])
+    print("unsorted vs sorted adjoint: normwise", rel)
+    assert rel <= 1e-5
+    assert np.array_equal(grads[0] != 0, grads[1] != 0)
+
+
+def _smooth_direction(ne, coords, seed):
+    """ne times a seeded smooth modulation m (|m| <= 1, three random low-k plane waves):
+    ne (1 +- eps m) stays non-negative where the field is clipped to zero."""
+    rng = np.random.default_rng(seed)
+    x, y, z = (c.astype(np.float64) for c in coords)
+    L = float(x[-1] - x[0])
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    m = np.zeros(ne.shape)
+    for _ in range(3):
+        k = rng.integers(1, 5, size=3) * (2 * np.pi / L)
+        m += rng.uniform(0.5, 1.0) * np.sin(k[0] * X + k[1] * Y + k[2] * Z
+                                            + rng.uniform(0, 2 * np.pi))
+    m /= np.abs(m).max()
+    return ne.astype(np.float64) * m
+
+
+def test_adjoint_full_config_directional_derivative():
+    """Config A in full: 256^3 Kolmogorov grid (10 mm), 2e6 rays at 532 nm, absorption,
+    checkpoints every 16 steps; loss = MSE of detector positions and energies against the
+    trace through a second seeded grid.  <dL/dn_e, delta> must equal the central
+    difference (L(ne + eps delta) - L(ne - eps delta)) / (2 eps) of the GPU forward for
+    three seeded smooth directions.  Positions are close to linear in ne at these
+    deflections and the loss is quadratic in them, so the central difference is
+    nearly exact even at eps = 1 %; every ray and every wavefront flush contributes."""
+    import torch
+
+    from sloth_amd.adjoint import detector_loss, trace_states
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import kolmogorov_field
+
+    L, n, lwl, cl = 10e-3, 256, 532e-9, 10.0
+    ne, coords = kolmogorov_field(n, L, seed=11, n0=2e24, frac=0.3, l_min=0.2e-3)
+    tne, _ = kolmogorov_field(n, L, seed=12, n0=2e24, frac=0.3, l_min=0.2e-3)
+    s0 = Beam(2_000_000, 4e-3, 0.0, L / 2, beam_type="square", seeded=True).s0
+    T = np.sqrt(8.0) * L / C
+    kw = dict(lwl=lwl, inv_brems=True, Te=20.0, Z=1.0, coulomb_log=cl,
+              save_times=np.array([0.0, T]), ckpt_every=16)
+    pos_scale = 1e-4
+    with torch.no_grad():
+        ts, ttau, _ = trace_states(torch.from_numpy(tne).cuda(), s0, *coords, **kw)
+        tp, te_ = ts[0:2].double(), torch.exp(-ttau)
+    del tne
+
+    def loss_of(ne_d):
+        with torch.no_grad():
+            st, ta, _ = trace_states(ne_d, s0, *coords, **kw)
+            return float(detector_loss(st, ta, tp, te_, pos_scale=pos_scale))
+
+    ne_t = torch.from_numpy(ne).cuda().requires_grad_(True)
+    state, tau, total = trace_states(ne_t, s0, *coords, **kw)
+    assert int(total.item()) > 1.0e9                       # 2e6 rays x ~510 in-box steps
+    detector_loss(state, tau, tp, te_, pos_scale=pos_scale).backward()
+    grad = ne_t.grad.double()
+    del state, tau
+    eps = 1e-2
+    for seed in (1, 2, 3):
+        delta = _smooth_direction(ne, coords, seed)
+        d_t = torch.from_numpy(delta).cuda()
+        ad = float((grad * d_t).sum())
+        base = torch.from_numpy(ne.astype(np.float64)).cuda()
+        lp = loss_of((base + eps * d_t).float())
+        lm = loss_of((base - eps * d_t).float())
+        fd = (lp - lm) / (2 * eps)
+        rel = abs(ad - fd) / abs(fd)
+        print(f"config A direction {seed}: <grad, delta> {ad:.6e}, central difference "
+              f"{fd:.6e}, relative {rel:.2e}", flush=True)
+        assert rel <= 2e-3

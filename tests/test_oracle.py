@@ -278,3 +278,30 @@ def test_jones_closed_form(orc):
     np.testing.assert_allclose(orc.jones(st, "z")[:, 0], [1.0, np.arctan(0.1), 2.0, np.arctan(-0.2)])
     np.testing.assert_allclose(orc.jones(st, "x")[:, 0], [2.0, np.arctan(-2.0), 3.0, np.arctan(10.0)])
     np.testing.assert_allclose(orc.jones(st, "y")[:, 0], [1.0, np.arctan(-0.5), 3.0, np.arctan(-5.0)])
+
+
+def test_face_gap_closed_forms(orc):
+    """The oracle's face gap (smallest distance of a stage point, the start point
+    excepted, to the surface of the closed box) on vacuum rays along z, where every
+    stage point is known: z = lo + j h c / 2 while the step touches the box."""
+    n = 9
+    c = _f32lin(-2e-3, 2e-3, n)
+    lo, hi = float(c[0]), float(c[-1])
+    zero = np.zeros((n, n, n))
+    hc = (hi - lo) / 7.3            # no stage point on a face
+    nsub = 12
+    # rays: centred; 0.4 mm inside the x = hi face; on the x = hi face; 0.1 mm outside it
+    xs = np.array([0.0, hi - 0.4e-3, hi, hi + 0.1e-3])
+    s0 = _rays(xs, np.zeros(4), lo)
+    _, steps, gap = orc.trace((zero, zero, zero), zero, (c, c, c), s0, [nsub], [hc / C],
+                              face_gap=True)
+    # z distances of the stage points of the steps that start inside (start point excepted)
+    zpts = [lo + j * hc / 2 for j in range(1, 2 * 8 + 1)]   # steps 0..7 start at z <= hi
+    zgap = min(min(z - lo, hi - z) if z <= hi else z - hi for z in zpts)
+    assert zgap > 1e-5
+    np.testing.assert_allclose(gap[0], zgap, rtol=1e-9)
+    np.testing.assert_allclose(gap[1], min(zgap, 0.4e-3), rtol=1e-9)
+    assert gap[2] == 0.0                 # on the face the whole way: inside, distance 0
+    assert np.isinf(gap[3]) and steps[3] == 0   # misses the box: no RK4 step at all
+    # the start point itself is excepted (it sits on the z = lo face for every ray)
+    assert gap[0] > 0
