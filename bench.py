@@ -13,11 +13,17 @@ detector histogram.  The packed grid and the rays are resident in HBM before
 timing starts.  value = in-box ray-steps per second (a ray-step is one RK4
 step with a stage point inside the grid box).
 
-Prints one JSON line (rank 0).  Also reports the roofline of the trace kernel
-(algorithmic bytes = 4 RHS x 8 corners x 4 channels x 4 B = 512 B per
-ray-step, over the kernel's average duration from HIP events on its stream)
-and a CPU baseline: the FP64 C oracle (OpenMP) on a bounded subsample of the
-same rays, on this host's cores.
+Prints one JSON line (rank 0).  ``roofline`` is the trace kernel's binding
+resource, VALU issue: its VALU lane-instructions per second (instructions per
+ray-step from the committed rocprofv3 PMC summary, times this run's ray-steps
+over the kernel's average duration from HIP events on its stream) against the
+rate of a calibration kernel of known VALU count (independent fp32 FMA
+chains, no memory traffic) timed in the same run.  ``hbm_roofline`` is the
+measured HBM traffic of one launch (FETCH_SIZE + WRITE_SIZE, PMC summary)
+over the same duration against 8 TB/s, beside the compulsory bytes (packed
+grid once, per-ray I/O once).  ``cpu_baseline`` is the FP64 C oracle built
+with -O3 -march=native on this host (OpenMP, all cores) on a bounded
+subsample of the same rays.
 """
 from __future__ import annotations
 
@@ -34,7 +40,14 @@ sys.path.insert(0, ROOT)
 
 C = 299792458.0
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_STEP = 4 * 8 * 4 * 4  # 4 RHS x 8 corners x 4 fp32 channels
+# algorithmic gather bytes of a ray-step: 4 RHS x 8 corners x 4 fp32 channels; served
+# from per-lane registers and L2/MALL, not HBM (reported, not a roof)
+BYTES_PER_STEP = 4 * 8 * 4 * 4
+# compulsory HBM bytes of one trace launch: the packed grid (16 B per voxel) once, and
+# per ray its start state (24 B) and order entry (4 B) read, final state (24 B), tau
+# (8 B), path (8 B) and step count (4 B) written
+VOXEL_BYTES = 16
+RAY_IO_BYTES = 24 + 4 + 24 + 8 + 8 + 4
 
 
 def log(*a):
@@ -63,25 +76,80 @@ def traffic_from_profiles():
     return profile_value("traffic_bytes_per_launch")
 
 
-def valu_roofline(steps_per_s_kernel, n_cu):
-    """VALU issue of the trace kernel, its binding resource: wave64 VALU instructions per
-    SIMD-cycle against the 0.5 a SIMD issues.  The instructions per ray-step and the clock
-    under load are PMC figures (SQ_INSTS_VALU, GRBM_GUI_ACTIVE) of the committed summary;
-    the ray-steps per second are this run's."""
+def valu_calibration(device, n_cu, iters=200_000, reps=3):
+    """VALU-issue rates measured on this GPU: slt_valu_calibrate for every instruction
+    form (SLT_CALIB_CHAINS independent chains per lane, eight wavefronts per SIMD), each
+    timed with HIP events, best of `reps` launches.  Returns ({form: lane-instructions per
+    second}, {form: ms})."""
+    import torch
+
+    from sloth_amd import _native as N
+
+    L = N.lib()
+    blocks = 8 * n_cu                       # 256-lane workgroups: 8 wavefronts per SIMD
+    out = torch.empty(blocks * 256, dtype=torch.float32, device=device)
+    stream = torch.cuda.current_stream(device)
+    lane_instr = blocks * 4 * N.SLT_CALIB_CHAINS * iters * 64.0
+    rates, times = {}, {}
+    for name, kind in N.SLT_CALIB_FORMS.items():
+        N.check(L.slt_valu_calibrate(kind, blocks, 1000, out.data_ptr(), N.stream_handle()))
+        best = None
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            N.check(L.slt_valu_calibrate(kind, blocks, iters, out.data_ptr(),
+                                         N.stream_handle()))
+            b.record(stream)
+            b.synchronize()
+            ms = a.elapsed_time(b)
+            best = ms if best is None else min(best, ms)
+        rates[name] = lane_instr / (best * 1e-3)
+        times[name] = best
+    return rates, times
+
+
+def valu_roofline(steps_per_s_kernel, rates, times=None):
+    """VALU issue of the trace kernel, its binding resource, in TOP/s of VALU
+    lane-instructions (a wave64 instruction = 64): the PMC summary's lane-instructions per
+    ray-step (SQ_INSTS_VALU x 64 / ray-steps) times the run's ray-steps per second of
+    kernel time, against the fastest instruction form of the calibration kernels measured
+    in the same run (``rates``: lane-instructions per second per form)."""
     lane_instr = profile_value("valu_lane_instr_per_ray_step")
-    ghz = profile_value("sclk_ghz_under_load")
-    if not lane_instr or not ghz or not n_cu:
+    if not lane_instr or not rates:
         return None
-    achieved = steps_per_s_kernel * lane_instr / 64.0 / (4 * n_cu * ghz * 1e9)
-    return {"bound": "valu_issue", "achieved": round(achieved, 4), "peak": 0.5,
-            "unit": "wave64 VALU instructions per SIMD-cycle", "frac": round(achieved / 0.5, 4),
-            "lane_instr_per_ray_step": lane_instr, "clock_ghz": ghz, "simds": 4 * n_cu,
-            "source": PMC_SUMMARY}
+    form = max(rates, key=rates.get)
+    achieved = steps_per_s_kernel * lane_instr / 1e12
+    peak = rates[form] / 1e12
+    return {"bound": "valu_issue", "achieved": round(achieved, 3), "peak": round(peak, 3),
+            "unit": "TOP/s", "frac": round(achieved / peak, 4),
+            "op": "VALU lane-instruction (one wave64 VALU instruction = 64)",
+            "lane_instr_per_ray_step": lane_instr, "instr_source": PMC_SUMMARY,
+            "peak_source": f"slt_valu_calibrate, fastest form ({form}), 16 independent chains "
+                           "per lane, 8 wavefronts per SIMD, timed in this run",
+            "calibration_tops": {k: round(v / 1e12, 3) for k, v in rates.items()},
+            "calibration_ms": {k: round(v, 3) for k, v in times.items()} if times else None}
+
+
+def hbm_roofline(kernel_s, n_voxels, n_rays):
+    """Measured HBM traffic of one trace launch (PMC summary) over the kernel's duration,
+    against the 8 TB/s peak, with the compulsory bytes beside it."""
+    traffic = traffic_from_profiles()
+    compulsory = VOXEL_BYTES * n_voxels + RAY_IO_BYTES * n_rays
+    if not traffic or not kernel_s:
+        return None
+    achieved = traffic / kernel_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "compulsory_bytes": compulsory,
+            "traffic_over_compulsory": round(traffic / compulsory, 3)}
 
 
 def cpu_baseline(ne, coords, s0, nsub, h, omega, target_s):
     from oracle import oracle as O
 
+    # the timed CPU restatement: the oracle's C source compiled for this host with
+    # -O3 -march=native (the checker build keeps its portable, contraction-free flags)
+    flags = O.load_native()
     cores = len(os.sched_getaffinity(0))
     c64 = [c.astype(np.float64) for c in coords]
     ne64 = ne.astype(np.float64)
@@ -92,18 +160,21 @@ def cpu_baseline(ne, coords, s0, nsub, h, omega, target_s):
     # calibrate on 2000 rays, then size the sample for ~target_s seconds
     pick = rng.choice(s0.shape[1], 2000, replace=False)
     t0 = time.perf_counter()
-    _, st = O.trace(grads, kap, c64, s0[:, pick].astype(np.float64), nsub, h64, nthreads=cores)
+    _, st = O.trace(grads, kap, c64, s0[:, pick].astype(np.float64), nsub, h64, nthreads=cores,
+                    native=True)
     dt = time.perf_counter() - t0
     rate = st.sum() / dt
     n = int(min(s0.shape[1], max(2000, target_s * rate / max(st.mean(), 1))))
     pick = rng.choice(s0.shape[1], n, replace=False)
     t0 = time.perf_counter()
-    _, st = O.trace(grads, kap, c64, s0[:, pick].astype(np.float64), nsub, h64, nthreads=cores)
+    _, st = O.trace(grads, kap, c64, s0[:, pick].astype(np.float64), nsub, h64, nthreads=cores,
+                    native=True)
     dt = time.perf_counter() - t0
     del grads, kap
     return {"value": float(st.sum() / dt), "unit": "ray-steps/s", "cores": cores, "kind": "port",
             "sample": f"{n} of the workload's rays (seeded random subset), same grid and "
-                      f"schedule, FP64 C oracle with OpenMP, {dt:.1f} s"}
+                      f"schedule, FP64 C oracle (gcc {flags}, OpenMP on {cores} threads), "
+                      f"{dt:.1f} s"}
 
 
 def serving_mix(n_requests, seed=12345):
@@ -243,6 +314,8 @@ def main():
                     help="requests in the serving mix (0 skips the serving leg)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (PMC calibration)")
+    ap.add_argument("--valu-calibration-only", action="store_true",
+                    help="run only the VALU calibration kernel (PMC calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the traced path runs on a single MI355X "
@@ -266,6 +339,13 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.valu_calibration_only:
+        rates, ms = valu_calibration(dev,
+                                     torch.cuda.get_device_properties(0).multi_processor_count)
+        for k in rates:
+            log(f"[bench] VALU calibration {k}: {rates[k] / 1e12:.2f} T lane-instr/s, "
+                f"{ms[k]:.3f} ms")
+        return
     L = 20e-3
     n = args.grid
     lwl = 532e-9
@@ -316,14 +396,16 @@ def main():
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
     ms_per_step = wall * 1e3 / args.steps
     value = total_steps / (ms_per_step * 1e-3)
-    achieved = total_steps * BYTES_PER_STEP / (kern_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(),
-                "kernel": "slt::trace_kernel<true,true>", "kernel_ms": round(kern_ms, 3),
-                "algorithmic_bytes_per_launch": total_steps * BYTES_PER_STEP}
-    valu = valu_roofline(total_steps / (kern_ms * 1e-3),
-                         torch.cuda.get_device_properties(0).multi_processor_count)
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    rates, calib_ms = valu_calibration(dev, n_cu)
+    roofline = valu_roofline(total_steps / (kern_ms * 1e-3), rates, calib_ms) or {}
+    roofline.update({"traffic": traffic_from_profiles(),
+                     "kernel": "slt::trace_kernel<true,true>", "kernel_ms": round(kern_ms, 3)})
+    hbm = hbm_roofline(kern_ms * 1e-3, n ** 3, args.rays)
+    gathers = {"bytes_per_ray_step": BYTES_PER_STEP,
+               "per_launch": total_steps * BYTES_PER_STEP,
+               "model": "4 RHS x 8 corners x 4 fp32 channels, served from per-lane "
+                        "registers and L2/MALL (not HBM traffic)"}
     log(f"[bench] {ms_per_step:.2f} ms/step, trace kernel {kern_ms:.2f} ms, "
         f"{value:.3e} ray-steps/s, detector sum {int(counts.sum())}")
     cpu = None
@@ -364,7 +446,8 @@ def main():
                    "accumulators": "fp64 tau and path"},
         "rhs_evals_per_s": 4.0 * value,   # RK4: 4 RHS evaluations per ray-step
         "roofline": roofline,
-        "valu_roofline": valu,
+        "hbm_roofline": hbm,
+        "algorithmic_gathers": gathers,
         "cpu_baseline": cpu,
         "serving": serving,
         "adjoint": adjoint,

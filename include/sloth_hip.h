@@ -387,6 +387,36 @@ int slt_adjoint_unpack(const float* grid_adj, const slt_grid_desc* g, const floa
                        double e_charge, double epsilon_0, double speed_of_light,
                        double coulomb_log, float* dne, slt_stream_t stream);
 
+/* ------------------------------------------------------------------------- */
+/* Measurement.  No reference counterpart (the reference times only wall      */
+/* clock around its solve, src/core/propagator.py:783,795).                   */
+/* ------------------------------------------------------------------------- */
+
+/* Independent chains per lane in the calibration kernels. */
+#define SLT_CALIB_CHAINS 16
+
+/* Instruction forms of the calibration kernels. */
+enum {
+    SLT_CALIB_FMA_SGPR = 0, /* v_fma_f32 (VOP3) with uniform (SGPR) multiplier and addend */
+    SLT_CALIB_FMA_VGPR = 1, /* v_fma_f32 (VOP3) with per-lane (VGPR) operands            */
+    SLT_CALIB_FMAC = 2,     /* v_fmac_f32 (VOP2), per-lane operands                        */
+    SLT_CALIB_ADD = 3,      /* v_add_f32 (VOP2)                                            */
+    SLT_CALIB_MUL = 4,      /* v_mul_f32 (VOP2)                                            */
+    SLT_CALIB_FMA64 = 5     /* v_fma_f64                                                   */
+};
+
+/*
+ * VALU-issue calibration: `blocks` workgroups of 256 lanes, each lane running
+ * SLT_CALIB_CHAINS independent chains of instruction form `kind` for `iters`
+ * iterations, so each wave64 issues SLT_CALIB_CHAINS * iters instructions of
+ * that form (and a few dozen other VALU instructions); no memory traffic but
+ * one float per lane stored to out (blocks * 256 floats, device).  The
+ * fastest form's instruction rate is the VALU-issue peak that the trace
+ * kernel's measured instruction rate is a fraction of.
+ */
+int slt_valu_calibrate(int32_t kind, int32_t blocks, int32_t iters, float* out,
+                       slt_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

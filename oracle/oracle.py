@@ -63,6 +63,29 @@ def build(verbose: bool = False) -> str:
     return _LIB
 
 
+# The timed CPU baseline: the same C source built for the host it runs on.  It is
+# compiled where it is used (the GPU box's host at bench time), into a temporary
+# directory, because -march=native code may not run on another CPU.
+NATIVE_FLAGS = ["-O3", "-march=native", "-std=gnu11", "-fopenmp", "-fPIC", "-shared"]
+_native = None
+
+
+def load_native() -> str:
+    """Build (once per process) and load the -O3 -march=native oracle; returns its flags.
+
+    GNU C mode lets gcc contract multiply-adds into FMAs, so this build is faster and
+    not bit-identical to the checker build; it is timed, never used as the checker."""
+    global _native
+    if _native is None:
+        import tempfile
+
+        d = tempfile.mkdtemp(prefix="sloth_oracle_native_")
+        out = os.path.join(d, "libsloth_oracle_native.so")
+        subprocess.run(["gcc", *NATIVE_FLAGS, _SRC, "-o", out, "-lm"], check=True)
+        _native = _bind(ctypes.CDLL(out))
+    return " ".join(NATIVE_FLAGS)
+
+
 def load():
     """Load (building first if absent) the C oracle."""
     global _lib
@@ -70,7 +93,11 @@ def load():
         return _lib
     if not os.path.exists(_LIB):
         build()
-    lib = ctypes.CDLL(_LIB)
+    _lib = _bind(ctypes.CDLL(_LIB))
+    return _lib
+
+
+def _bind(lib):
     P = ctypes.POINTER
     d = ctypes.c_double
     i32, i64 = ctypes.c_int, ctypes.c_int64
@@ -81,7 +108,6 @@ def load():
                               i64, P(d), i32, P(ctypes.c_int32), P(d), i32, P(d),
                               P(ctypes.c_int64), P(d), i32]
     lib.orc_max_threads.restype = ctypes.c_int
-    _lib = lib
     return lib
 
 
@@ -150,16 +176,20 @@ def interp_points(coords, values, pts):
     return out
 
 
-def trace(grads, kap, coords, s0, nsub, h, naive=False, nthreads=0, face_gap=False):
+def trace(grads, kap, coords, s0, nsub, h, naive=False, nthreads=0, face_gap=False,
+          native=False):
     """Fixed-step RK4 trace.  Returns (saves[(S+1),8,N], steps[N]), and with
     ``face_gap=True`` also gap[N]: per ray, the smallest distance [m] from any stage
     point of its RK4 steps (the start point excepted) to the surface of the closed
     grid box.
 
     Rows of each save: x, y, z, vx, vy, vz, tau, path (length of the polyline through
-    the step end points).
+    the step end points).  ``native=True`` runs the -march=native build (``load_native``,
+    the timed CPU baseline) instead of the checker build.
     """
-    lib = load()
+    if native:
+        load_native()
+    lib = _native if native else load()
     dndx, dndy, dndz = (_f64(g) for g in grads)
     kp = _f64(kap) if kap is not None else None
     cx, cy, cz = (_f64(c) for c in coords)

@@ -22,11 +22,14 @@ EXPORTED = (
     "slt_kappa", "slt_trace", "slt_jones", "slt_histogram2d", "slt_morton_keys",
     "slt_ray_state_bytes", "slt_ray_state_init", "slt_job_table_prepare", "slt_advance",
     "slt_checkpoint_bytes", "slt_tape_bytes", "slt_forward_ckpt", "slt_adjoint",
-    "slt_adjoint_unpack", "slt_rtm", "slt_beam", "slt_adjoint_noalloc",
+    "slt_adjoint_unpack", "slt_rtm", "slt_beam", "slt_adjoint_noalloc", "slt_valu_calibrate",
 )
 
 SLT_MAX_JOBS = 64
 SLT_ADJ_SCRATCH_BYTES = 128 << 20
+SLT_CALIB_CHAINS = 16
+SLT_CALIB_FORMS = {"fma_sgpr": 0, "fma_vgpr": 1, "fmac": 2, "add_f32": 3, "mul_f32": 4,
+                   "fma_f64": 5}
 SLT_WORK_QUEUES = 8
 SLT_MAX_OPTICS = 32
 SLT_OPTIC_TRAVEL = 0
@@ -177,6 +180,7 @@ def lib():
     L.slt_adjoint.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(AdjointArgs), vp]
     L.slt_adjoint_noalloc.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(AdjointArgs),
                                       vp]
+    L.slt_valu_calibrate.argtypes = [i32, i32, i32, vp, vp]
     L.slt_adjoint_unpack.argtypes = [vp, ctypes.POINTER(GridDesc), vp, vp, vp, vp, dbl, vp, i64,
                                      vp, i64, dbl, dbl, dbl, dbl, dbl, vp, vp]
     for name in EXPORTED:

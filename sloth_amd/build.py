@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libsloth_hip.so")
-SOURCES = ["abi.hip", "grid_pack.hip", "trace.hip", "detector.hip", "optics.hip", "beam.hip", "adjoint.hip"]
+SOURCES = ["abi.hip", "grid_pack.hip", "trace.hip", "detector.hip", "optics.hip", "beam.hip",
+           "adjoint.hip", "calib.hip"]
 ARCH = os.environ.get("SLOTH_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 VALU forms are disabled for all device code (some of their forms

@@ -253,3 +253,10 @@ def test_sizes_of_invalid_requests_are_zero(L):
     assert L.slt_checkpoint_bytes(10, 10, 0) == 0
     assert L.slt_tape_bytes(10, 0) == 0
     assert L.slt_ray_state_bytes(10) > 0 and L.slt_tape_bytes(10, 16) == 16 * 7 * 4 * 10
+
+
+def test_valu_calibrate_rejections(L):
+    for kind, blocks, iters, out in ((0, 0, 10, FAKE), (1, 8, 0, FAKE), (2, 8, 10, None),
+                                     (0, -1, 10, FAKE), (6, 8, 10, FAKE), (-1, 8, 10, FAKE)):
+        _expect(L, L.slt_valu_calibrate(kind, blocks, iters, out, None), N.SLT_ERR_INVALID,
+                "slt_valu_calibrate")

@@ -1,5 +1,6 @@
 """bench.py pieces that run without a GPU: the one-GPU guard, the figures read from the
-committed rocprofv3 summaries, the VALU-issue roofline arithmetic and the serving mix."""
+committed rocprofv3 summaries, the VALU-issue and HBM roofline arithmetic and the serving
+mix."""
 import json
 import os
 import subprocess
@@ -27,27 +28,38 @@ def test_more_than_one_gpu_exits_with_a_message():
 def test_profile_figures_present_and_consistent():
     """The committed PMC summary carries the figures the bench line quotes."""
     lane = bench.profile_value("valu_lane_instr_per_ray_step")
-    ghz = bench.profile_value("sclk_ghz_under_load")
     traffic = bench.traffic_from_profiles()
     assert lane is not None and 150 < lane < 500
-    assert ghz is not None and 1.0 < ghz < 2.5
     assert traffic is not None and traffic > 0
     assert bench.profile_value("no_such_key") is None
 
 
 def test_valu_roofline_arithmetic():
     lane = bench.profile_value("valu_lane_instr_per_ray_step")
-    ghz = bench.profile_value("sclk_ghz_under_load")
-    n_cu = 256
-    # the step rate at which the kernel would issue exactly 0.5 wave64 VALU per SIMD-cycle
-    full = 0.5 * 4 * n_cu * ghz * 1e9 * 64.0 / lane
-    r = bench.valu_roofline(full, n_cu)
+    rates = {"fma_sgpr": 3.3e13, "add_f32": 8.0e13, "fma_f64": 1.7e13}   # lane-instr/s
+    # the step rate at which the kernel issues exactly the fastest form's rate
+    full = 8.0e13 / lane
+    r = bench.valu_roofline(full, rates, {"fma_sgpr": 50.0, "add_f32": 20.0, "fma_f64": 99.0})
+    assert r["bound"] == "valu_issue" and r["unit"] == "TOP/s"
     assert r["frac"] == pytest.approx(1.0, abs=1e-4)
-    assert r["simds"] == 1024 and r["peak"] == 0.5
-    r = bench.valu_roofline(0.25 * full, n_cu)
+    assert r["peak"] == pytest.approx(80.0) and "add_f32" in r["peak_source"]
+    assert r["calibration_tops"]["fma_sgpr"] == pytest.approx(33.0)
+    r = bench.valu_roofline(0.25 * full, rates)
     assert r["frac"] == pytest.approx(0.25, abs=1e-4)
     json.dumps(r)  # goes into the JSON line as is
-    assert bench.valu_roofline(full, 0) is None
+    assert bench.valu_roofline(full, {}) is None
+
+
+def test_hbm_roofline_arithmetic():
+    traffic = bench.traffic_from_profiles()
+    n_vox, n_rays = 512 ** 3, 10_000_000
+    r = bench.hbm_roofline(traffic / 8.0e12, n_vox, n_rays)   # duration at exactly 8 TB/s
+    assert r["frac"] == pytest.approx(1.0, abs=1e-4) and r["unit"] == "GB/s"
+    # compulsory: 16 B per voxel + 72 B of per-ray I/O
+    assert r["compulsory_bytes"] == 16 * n_vox + 72 * n_rays
+    assert r["traffic_over_compulsory"] == pytest.approx(traffic / r["compulsory_bytes"],
+                                                         rel=1e-3)
+    json.dumps(r)
 
 
 def test_algorithmic_bytes_per_ray_step():
