@@ -1,4 +1,6 @@
-"""Pins of the random-field oracle (oracle/fields.py) and host tests of the VTK image files."""
+"""Pins of the random-field oracle (oracle/fields.py): the reference generators' own outputs
+(tests/golden/fields_vectors.npz) and properties the constructions fix; host tests of the
+VTK image files."""
 import base64
 import os
 import zlib
@@ -11,6 +13,53 @@ from oracle import fields as F
 
 def _k53(k):
     return k ** (-5 / 3)
+
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fields_vectors.npz")
+
+
+def _k113(k):
+    return k ** (-11.0 / 3.0)
+
+
+def golden_fields():
+    """The reference's own fields (tests/golden/make_fields_golden.py ran
+    src/field_generator/gaussian3D.py) with the arguments and seeds that made them."""
+    z = np.load(GOLD)  # plain arrays; allow_pickle stays False
+    cos_args = ([float(z[f"cos_{a}"]) for a in ("lx", "ly", "lz")]
+                + [int(z[f"cos_{a}"]) for a in ("nx", "ny", "nz", "nmodes")]
+                + [float(z["cos_wn1"])])
+    dom = [([float(z[f"domain{i}_{a}"]) for a in ("l_max", "l_min", "extent")]
+            + [int(z[f"domain{i}_res"]), float(z[f"domain{i}_factor"])]) for i in (0, 1)]
+    return z, cos_args, dom
+
+
+def test_oracle_reproduces_the_reference_fields():
+    """The oracle's cos / fft / domain_fft equal the reference generators' outputs on the
+    same arguments and seeds: the spectral amplitudes (A_m of the mode sum, sqrt of the
+    spectrum in both FFT constructions), the draw order and the normalisation are the
+    reference's."""
+    z, cos_args, dom = golden_fields()
+    np.random.seed(int(z["cos_seed"]))
+    r, _ = F.cos(_k113, *cos_args)
+    np.testing.assert_allclose(r, z["cos_field"], rtol=1e-12, atol=1e-12 * np.abs(r).max())
+    np.random.seed(int(z["fft_seed"]))
+    s = F.fft(_k113, int(z["fft_N"]))
+    np.testing.assert_allclose(s.real, z["fft_field"], rtol=1e-12,
+                               atol=1e-12 * np.abs(s.real).max())
+    for i, args in enumerate(dom):
+        f, _, _ = F.domain_fft(_k113, *args)
+        np.testing.assert_allclose(f, z[f"domain{i}_field"], rtol=1e-12, atol=1e-13)
+    # the vectors do constrain the amplitude: the field of the spectrum itself instead of
+    # its square root (a plausible slip) is far from the reference's
+    k, band = F.domain_fft(_k113, *dom[0])[1:]
+    S = np.zeros_like(k)
+    S[band] = _k113(k[band])
+    np.random.seed(42)
+    noise = np.random.normal(0, 1, k.shape) + 1j * np.random.normal(0, 1, k.shape)
+    wrong = np.fft.ifftn(noise * S).real
+    wrong /= np.abs(wrong).max()
+    assert np.abs(wrong - z["domain0_field"]).max() > 0.1
 
 
 def test_domain_fft_is_band_limited_normalised_and_seeded():

@@ -55,6 +55,28 @@ def test_fft_and_cos_match_oracle():
     _close(c, d, rtol=1e-11)
 
 
+def test_generators_match_the_reference_fields():
+    """The GPU generators against the reference's own outputs (same arguments, same seeds),
+    at fp64 FFT and cosine rounding."""
+    from sloth_amd.field_generator.gaussian3D import gaussian3D
+
+    from test_fields_oracle import _k113, golden_fields
+
+    z, cos_args, dom = golden_fields()
+    g = gaussian3D(_k113)
+    np.random.seed(int(z["cos_seed"]))
+    _close(g.cos(*cos_args), z["cos_field"], rtol=1e-11)
+    np.random.seed(int(z["fft_seed"]))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        _close(g.fft(int(z["fft_N"])), z["fft_field"])
+    for i, args in enumerate(dom):
+        f = g.domain_fft(*args)
+        assert f.shape == z[f"domain{i}_field"].shape
+        _close(f, z[f"domain{i}_field"])
+        np.testing.assert_array_equal(g.xc, z[f"domain{i}_x"])
+        np.testing.assert_array_equal(g.zc, z[f"domain{i}_z"])
+
+
 def test_turbulent_box_spherical_and_step_domains():
     import torch
 

@@ -259,3 +259,80 @@ def test_vacuum_is_exact():
     s = s0.astype(np.float64)
     np.testing.assert_allclose(fin[:3], s[:3] + s[3:6] * T, rtol=0, atol=2e-9)
     np.testing.assert_array_equal(fin[3:6], s[3:6])
+
+
+def test_solve_reference_example_exponential_cos(orc):
+    """The reference's example notebook (examples/01.example_sinewave.ipynb, the setup of its
+    only timed run): ScalarDomain(2 (5, 5, 10) mm, 128, ne_type="test_exponential_cos")
+    probed along z, Beam(2e6, 5 mm, divergence 5e-5, 10 mm, "circular") at 1064 nm,
+    solve(beam.s0, domain, 10 mm), then the notebook's Refractometry, Shadowgraphy and
+    Schlieren chains.  A seeded 2000-ray subset of rf is checked against the oracle's Jones
+    vectors of the same rays traced through the same densities (relative to max(|x|, the
+    half extent) in position, in rad in angle; bar below)."""
+    from sloth_amd.core import engine
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.core.domain import ScalarDomain
+    from sloth_amd.core.propagator import solve
+    from sloth_amd.processing import diagnostics as diag
+
+    lwl, n_rays = 1064e-9, 2_000_000
+    ext = np.array([5e-3, 5e-3, 10e-3])
+    domain = ScalarDomain(2 * ext, 128, ne_type="test_exponential_cos", probing_direction="z")
+    beam = Beam(n_rays, ext[0], 5e-5, ext[2], probing_direction="z", beam_type="circular",
+                seeded=True)
+    rf, amp, duration = solve(beam.s0, domain, ext[2], lwl=lwl, verbose=False)
+    assert rf.shape == (4, n_rays) and amp is None and duration > 0
+    assert np.isfinite(rf).all()
+
+    omega = 2 * np.pi * C / lwl
+    grid = engine.PackedGrid(domain.ne, domain.x, domain.y, domain.z, omega)
+    nsub, h = engine.schedule(np.array([0.0, np.sqrt(8.0) * ext[2] / C]),
+                              engine.nominal_step(grid, 0.5))
+    idx = np.sort(np.random.default_rng(101).choice(n_rays, 2000, replace=False))
+    c64 = [c.astype(np.float64) for c in (domain.x, domain.y, domain.z)]
+    grads = orc.gradients(domain.ne.astype(np.float64), *c64, omega)
+    h64 = h.astype(np.float32).astype(np.float64)
+    s0 = beam.s0[:, idx].astype(np.float64)
+    saves, _ = orc.trace(grads, None, c64, s0, nsub, h64)
+    ref = orc.jones(saves[-1][:6], "z")
+    # This domain refracts hard (0.64 of the critical density, density 10x per 2 mm in x,
+    # a 1 mm period in y; rays turn by up to 0.5 rad), so the answer itself is sensitive:
+    # moving the start positions by one fp32 ulp moves the FP64 oracle's own outputs by up
+    # to ~3e-5.  Per ray, the bar is 1e-5 plus 8x that one-ulp sensitivity (an fp32 trace
+    # rounds at every one of its ~500 steps); rays whose sensitivity is below 1e-7 are held
+    # to 1e-5 outright.
+    s1 = s0.copy()
+    for r in (0, 1):
+        s1[r] = np.nextafter(s0[r].astype(np.float32), np.float32(np.inf)).astype(np.float64)
+    moved, _ = orc.trace(grads, None, c64, s1, nsub, h64)
+    ref1 = orc.jones(moved[-1][:6], "z")
+
+    def rel(a, b):
+        out = []
+        for row in range(4):
+            d = np.abs(a[row] - b[row])
+            out.append(d / np.maximum(np.abs(b[row]), ext[row // 2]) if row in (0, 2) else d)
+        return np.max(out, axis=0)
+
+    got = rf[:, idx].astype(np.float64)
+    err = rel(got, ref)
+    sens = rel(ref1, ref)
+    well = sens < 1e-7
+    print(f"example notebook: max error {err.max():.2e} (well-conditioned rays "
+          f"{err[well].max():.2e}, {well.sum()} of {well.size}); one-ulp sensitivity max "
+          f"{sens.max():.2e}; rays over 1e-5: {(err > 1e-5).sum()}", flush=True)
+    assert well.mean() > 0.5
+    assert err[well].max() <= 1e-5
+    assert np.all(err <= 1e-5 + 8 * sens), np.max(err - 8 * sens)
+    # the beam is refracted: the perturbation deflects rays by far more than the tolerance
+    assert np.abs(ref[1]).max() > 1e-3
+
+    # the notebook passes (lwl, rf); the diagnostics take (rf, ...) in the reference's
+    # current src/processing/diagnostics.py:177
+    for d, solve_chain in ((diag.Refractometry(rf), "incoherent_solve"),
+                           (diag.Shadowgraphy(rf), "single_lens_solve"),
+                           (diag.Schlieren(rf), "DF_solve")):
+        getattr(d, solve_chain)()
+        d.histogram(bin_scale=1, clear_mem=False)
+        H = np.asarray(d.H)
+        assert H.ndim == 2 and 0 < H.sum() <= n_rays, (solve_chain, H.sum())

@@ -140,6 +140,70 @@ def test_scalar_domain_protocol():
         ScalarDomain([4e-3] * 3, [8] * 3, inv_brems=True, Te=None, Z=1.0)
 
 
+def test_scalar_domain_accepts_only_the_reference_keywords():
+    """The reference constructor's keyword list (src/core/domain.py:57-59): a misspelt
+    keyword (e.g. inverse_brems for inv_brems) is a TypeError, not a silent no-op; an
+    unknown ne_type and a malformed lengths/dims raise as there (:232-258, :522-566)."""
+    from sloth_amd.core.domain import ScalarDomain
+
+    ScalarDomain(1e-3, 8, auto_batching=False, iteration=1, region_count=1, leeway_factor=1.2,
+                 extra_info=False, memory_reporting=False, memory_limit=None, Np=10, s=1.0,
+                 s1=0.1, s2=0.1, Ly=1e-3, ne_0=1e24, coulomb_log=10.0)
+    for bad in ("inverse_brems", "Te_eV", "ne0", "S1"):
+        with pytest.raises(TypeError):
+            ScalarDomain(1e-3, 8, **{bad: 1.0})
+    with pytest.raises(AssertionError, match="No valid profile"):
+        ScalarDomain(1e-3, 8, ne_type="test_B")
+    with pytest.raises(Exception, match="lengths must have len = 3"):
+        ScalarDomain([1e-3, 1e-3], 8)
+    with pytest.raises(Exception, match="n must have len = 3"):
+        ScalarDomain(1e-3, [8, 8])
+    d = ScalarDomain(2e-3, [6, 7, 8])
+    assert d.ne.shape == (6, 7, 8) and not d.ne.any()          # no ne_type: empty domain
+
+
+def test_scalar_domain_generated_profiles():
+    """The generated profiles (src/core/domain.py:566-683), pinned by what their formulas
+    fix: test_slab is linear in x through ne_0 at x = 0; test_linear_cos is linear in x at
+    every y, periodic in y with period Ly, ne_0 (1 + s2) at x = 0 on a crest, and
+    independent of z; test_exponential_cos grows by 10 per distance s in x, vanishes on
+    troughs and is 2 ne_0 10^(x/s) on crests.  All in float32, as the reference computes
+    them."""
+    from sloth_amd.core.domain import ScalarDomain
+
+    d = ScalarDomain([4e-3, 4e-3, 2e-3], [9, 5, 3], ne_type="test_slab", s=0.5, ne_0=1e24)
+    assert d.ne.dtype == np.float32
+    x = d.x.astype(np.float64)
+    np.testing.assert_allclose(d.ne[:, 2, 1], 1e24 * (1 + 0.5 * x / 4e-3), rtol=1e-6)
+    assert d.ne[4, 0, 0] == np.float32(1e24)                   # x = 0
+
+    Ly = 1e-3
+    d = ScalarDomain([4e-3, 4e-3, 2e-3], [9, 17, 4], ne_type="test_linear_cos", s1=0.3, s2=0.2,
+                     ne_0=2e23, Ly=Ly)
+    ne = d.ne.astype(np.float64)
+    assert np.array_equal(d.ne[..., 0], d.ne[..., 3])          # no z dependence
+    second = ne[2:] - 2 * ne[1:-1] + ne[:-2]                   # linear in x
+    assert np.abs(second).max() < 1e-5 * ne.max()
+    # y spacing is 0.25 mm, so 4 cells = one period of 1 mm
+    np.testing.assert_allclose(ne[:, 4:], ne[:, :-4], rtol=1e-5)
+    yi = int(np.argmin(np.abs(np.cos(2 * np.pi * d.y.astype(np.float64) / Ly) - 1.0)))
+    assert ne[4, yi, 0] == pytest.approx(2e23 * 1.2, rel=1e-6)
+
+    s = -2e-3
+    d = ScalarDomain([10e-3, 2e-3, 1e-3], [11, 9, 2], ne_type="test_exponential_cos")
+    ne = d.ne.astype(np.float64)
+    x = d.x.astype(np.float64)
+    crest = 4                                                  # y = 0: cos = 1
+    assert d.y[crest] == 0
+    np.testing.assert_allclose(ne[:, crest, 0], 2e24 * 10 ** (x / s), rtol=2e-6)
+    np.testing.assert_allclose(ne[1:, crest, 0] / ne[:-1, crest, 0],
+                               10 ** ((x[1] - x[0]) / s), rtol=2e-6)
+    trough = 0                                                 # y = -1 mm = -Ly: cos = 1;
+    assert ne[:, trough, 0] == pytest.approx(ne[:, crest, 0], rel=1e-5)
+    half = 2                                                   # y = -0.5 mm: cos = -1
+    assert np.abs(ne[:, half, 0]).max() < 1e-6 * ne.max()
+
+
 def _vac(n=12, half=2e-3):
     class D:
         pass
