@@ -2,8 +2,10 @@
 
 Requests (``RequestSpec``) arrive over time.  The engine
 
-* admits them FIFO while a slot is free and their HBM footprint fits the
-  budget (a request larger than the whole budget is rejected at once),
+* admits them in arrival order while a slot is free and their HBM footprint
+  fits the budget, letting a request that fits pass an older one that waits
+  for memory unless that one has waited longer than an age cap (a request
+  larger than the whole budget is rejected at once),
 * sets each admitted request up on a side stream (field synthesis, k1 pack,
   beam, Morton order, resumable ray state) while earlier requests keep
   running,
@@ -42,31 +44,46 @@ from .requests import RequestSpec, materialize
 # ---------------------------------------------------------------------------
 
 class Admission:
-    """FIFO admission by free slots and HBM footprint."""
+    """Admission by free slots and HBM footprint, in arrival order, without head-of-line
+    blocking: an arrived request that fits is admitted even when an older one is waiting
+    for memory, unless the oldest waiting request has waited longer than ``max_wait``
+    seconds -- then nothing behind it is admitted until it is, so running requests drain
+    and free its memory.  A request larger than the whole budget is rejected at once."""
 
-    def __init__(self, budget_bytes: int, max_slots: int):
+    def __init__(self, budget_bytes: int, max_slots: int, max_wait: float = 0.05):
         if max_slots < 1:
             raise ValueError("max_slots must be >= 1")
+        if max_wait < 0:
+            raise ValueError("max_wait must be >= 0")
         self.budget = int(budget_bytes)
         self.max_slots = int(max_slots)
+        self.max_wait = float(max_wait)
         self.used = 0
         self.slots = 0
 
     def admit(self, queue: deque, now: float):
-        """Pop the arrived head-of-queue requests that fit.  Returns (admitted, rejected)."""
+        """Pop the arrived requests that fit.  Returns (admitted, rejected)."""
         admitted, rejected = [], []
-        while queue and queue[0].arrival <= now:
-            req = queue[0]
+        i = 0
+        while i < len(queue) and queue[i].arrival <= now:
+            req = queue[i]
             fp = req.footprint_bytes()
             if fp > self.budget:
-                rejected.append(queue.popleft())
+                rejected.append(req)
+                del queue[i]
                 continue
-            if self.slots >= self.max_slots or self.used + fp > self.budget:
+            if self.slots >= self.max_slots:
                 break
-            queue.popleft()
-            self.used += fp
-            self.slots += 1
-            admitted.append(req)
+            if self.used + fp <= self.budget:
+                del queue[i]
+                self.used += fp
+                self.slots += 1
+                admitted.append(req)
+                continue
+            # req waits for memory; later arrivals may pass it while it is young
+            if now - req.arrival > self.max_wait:
+                break
+            i += 1
         return admitted, rejected
 
     def release(self, req: RequestSpec):
@@ -128,6 +145,8 @@ class ServingEngine:
 
     ``blocks``: workgroups of the persistent step-batch kernel; 1024 = one resident
     round on MI355X (256 CUs x 4 SIMDs x 4 wavefronts at its register count).
+    ``max_wait``: seconds an arrived request may wait for memory while younger requests
+    that fit are admitted past it; after that, admission holds everything behind it.
     ``batch_work``: target ray-steps per batch (the batch length in steps is this
     over the rays resident, at most 4096); smaller batches admit new requests sooner,
     but each batch costs ~0.35 ms of unit dealing and state traffic whatever its
@@ -136,7 +155,8 @@ class ServingEngine:
     """
     def __init__(self, *, max_slots: int = 16, hbm_budget: int | None = None,
                  blocks: int = 1024, batch_work: float = 4e9, use_graph: bool = True,
-                 step_fraction: float = 0.5, keep_outputs: bool = True, device=None):
+                 step_fraction: float = 0.5, keep_outputs: bool = True, max_wait: float = 0.05,
+                 device=None):
         self.dev = device if device is not None else N.require_device()
         if hbm_budget is None:
             free, _total = torch.cuda.mem_get_info(self.dev)
@@ -151,6 +171,7 @@ class ServingEngine:
         self.fill_rays = 2 * 64 * (self.blocks * 4)
         self.step_fraction = step_fraction
         self.keep_outputs = keep_outputs
+        self.max_wait = float(max_wait)
         self.main = torch.cuda.Stream(self.dev)
         self.setup_stream = torch.cuda.Stream(self.dev)
         tb = ctypes.sizeof(N.JobTable)
@@ -315,7 +336,7 @@ class ServingEngine:
         if not realtime:
             for r in queue:
                 r.arrival = 0.0
-        adm = Admission(self.budget, self.max_slots)
+        adm = Admission(self.budget, self.max_slots, self.max_wait)
         responses: dict[int, Response] = {}
         active: list[_Active] = []
         pending = deque()   # admitted, not yet set up

@@ -79,7 +79,9 @@ def test_admission_by_footprint_and_rejection():
     reqs = _mix(5, seed=2)
     budget = max(r.footprint_bytes() for r in reqs)  # one request at a time fits
     huge = RequestSpec(rid=99, field=FieldSpec(side=256), beam=BeamSpec(n_rays=1000))
-    eng = ServingEngine(max_slots=8, hbm_budget=budget, blocks=128, batch_work=1e6)
+    # max_wait = 0: strict arrival order (no request passes one waiting for memory)
+    eng = ServingEngine(max_slots=8, hbm_budget=budget, blocks=128, batch_work=1e6,
+                        max_wait=0.0)
     resp, stats = eng.run(reqs + [huge], realtime=False)
     assert resp[99].status == "rejected" and "budget" in resp[99].message
     assert stats["ok"] == 5 and stats["rejected"] == 1
@@ -200,3 +202,35 @@ def test_request_field_statistics_and_shared_spectrum():
     outside = (k < 0.95 * 2 * np.pi / L) | (k > 1.05 * 2 * np.pi / lmin)
     outside[0, 0, 0] = False
     assert F[outside].max() <= 1e-5 * F.max()
+
+
+@pytest.mark.parametrize("n", [48, 33])
+def test_half_spectrum_noise_is_hermitian_with_flat_power(n):
+    """The serving field's k-space noise: on the kz = 0 plane (and kz = n/2 for even n) each
+    mode equals the conjugate of its partner (-kx, -ky), so the self-conjugate modes are
+    real; every plane carries the same mean power E|F|^2 = 2 as the untouched ones."""
+    import torch
+
+    from sloth_amd.serving.requests import _half_spectrum_noise
+
+    dev = torch.device("cuda", 0)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    F = _half_spectrum_noise(n, gen, dev).cpu().numpy().astype(np.complex128)
+    assert F.shape == (n, n, n // 2 + 1)
+    neg = (-np.arange(n)) % n
+    planes = (0, n // 2) if n % 2 == 0 else (0,)
+    for kz in planes:
+        P = F[:, :, kz]
+        np.testing.assert_allclose(P, np.conj(P[neg][:, neg]), rtol=0, atol=1e-6)
+        for i in (0, n // 2) if n % 2 == 0 else (0,):
+            for j in (0, n // 2) if n % 2 == 0 else (0,):
+                assert P[i, j].imag == 0.0
+    power = (np.abs(F) ** 2).mean(axis=(0, 1))
+    # n^2 modes per plane: the mean of |F|^2 (exponential, mean 2) is within ~4 % at 3 sigma
+    np.testing.assert_allclose(power, 2.0, rtol=6.0 / n)
+    for kz in planes:
+        assert abs(power[kz] / power[1:n // 2].mean() - 1.0) < 6.0 / n
+    # the real inverse transform of this spectrum is the full complex one's real part
+    full = np.fft.irfftn(F, s=(n, n, n))
+    assert np.isfinite(full).all()

@@ -42,6 +42,38 @@ def test_admission_rejects_oversized_and_keeps_going():
     assert adm.admit(q2, 0.0)[0][0].rid == 2
 
 
+def test_admission_passes_a_blocked_head_until_its_age_cap():
+    """A request that fits is admitted past an older one waiting for memory (no head-of-line
+    blocking) while that one is younger than max_wait; once it is older, nothing behind it
+    is admitted, and it goes first as soon as memory frees."""
+    running = _req(0, side=128)
+    head = _req(1, side=192, arrival=0.0)           # does not fit beside `running`
+    s1, s2 = _req(2, arrival=0.01), _req(3, arrival=0.02)
+    budget = head.footprint_bytes() + s1.footprint_bytes() // 2
+    adm = Admission(budget, max_slots=8, max_wait=0.05)
+    assert adm.admit(deque([running]), 0.0)[0] == [running]
+    q = deque([head, s1, s2])
+    got, rej = adm.admit(q, now=0.03)               # head waits 30 ms: small ones pass it
+    assert [r.rid for r in got] == [2, 3] and not rej and list(q) == [head]
+    adm.release(s1)
+    adm.release(s2)
+    s3 = _req(4, arrival=0.04)
+    q.append(s3)
+    got, _ = adm.admit(q, now=0.06)                 # head has waited 60 ms: hold s3
+    assert got == [] and list(q) == [head, s3]
+    adm.release(running)                            # memory frees: head first, then s3 fits
+    got, _ = adm.admit(q, now=0.07)
+    assert [r.rid for r in got] == [1] and list(q) == [s3]
+    assert adm.used == head.footprint_bytes() and adm.slots == 1
+    # an oversized request anywhere in the arrived queue is rejected without blocking
+    huge = _req(5, side=384, rays=4_000_000, arrival=0.08)
+    q = deque([huge, _req(6, arrival=0.09)])
+    got, rej = Admission(head.footprint_bytes(), 4).admit(q, now=0.1)
+    assert rej == [huge] and [r.rid for r in got] == [6]
+    with pytest.raises(ValueError):
+        Admission(1, 1, max_wait=-1.0)
+
+
 def _grid_desc(n=8):
     c = np.float32(np.linspace(0, 1e-3, n))
     d = N.GridDesc()
