@@ -316,6 +316,8 @@ def main():
                     help="run one plain device copy of this many MiB first (PMC calibration)")
     ap.add_argument("--valu-calibration-only", action="store_true",
                     help="run only the VALU calibration kernel (PMC calibration) and exit")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint leg (profiling) and print its figures")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the traced path runs on a single MI355X "
@@ -339,6 +341,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.adjoint_only:
+        log(json.dumps(adjoint_bench()))
+        return
     if args.valu_calibration_only:
         rates, ms = valu_calibration(dev,
                                      torch.cuda.get_device_properties(0).multi_processor_count)

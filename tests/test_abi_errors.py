@@ -227,8 +227,13 @@ def test_adjoint_rejections(L):
         _expect(L, noalloc(FAKE, ctypes.byref(g),
                            ctypes.byref(aargs(tape=None, **{missing: None})), None),
                 N.SLT_ERR_INVALID, "slt_adjoint_noalloc")
-    cap = N.SLT_ADJ_SCRATCH_BYTES // 28
-    for n, K in ((cap // 16 + 1, 16), (cap + 1, 1), (2, cap), (2 ** 37, 2 ** 30)):
+    for K in (1, 4, 16, 1000):
+        n = N.SLT_ADJ_SCRATCH_BYTES // int(L.slt_tape_bytes(1, K)) + 1
+        assert int(L.slt_tape_bytes(n, K)) > N.SLT_ADJ_SCRATCH_BYTES
+        _expect(L, noalloc(FAKE, ctypes.byref(g),
+                           ctypes.byref(aargs(tape=None, n_rays=n, ckpt_every=K)), None),
+                N.SLT_ERR_UNSUPPORTED, "scratch")
+    for n, K in ((2 ** 37, 2 ** 30),):
         _expect(L, noalloc(FAKE, ctypes.byref(g),
                            ctypes.byref(aargs(tape=None, n_rays=n, ckpt_every=K)), None),
                 N.SLT_ERR_UNSUPPORTED, "scratch")
@@ -252,7 +257,10 @@ def test_sizes_of_invalid_requests_are_zero(L):
     assert L.slt_checkpoint_bytes(-1, 10, 16) == 0
     assert L.slt_checkpoint_bytes(10, 10, 0) == 0
     assert L.slt_tape_bytes(10, 0) == 0
-    assert L.slt_ray_state_bytes(10) > 0 and L.slt_tape_bytes(10, 16) == 16 * 7 * 4 * 10
+    assert L.slt_ray_state_bytes(10) > 0
+    # 28 bytes per ray and recorded step; at most a fixed number of steps are recorded
+    assert L.slt_tape_bytes(10, 1) == 28 * 10
+    assert L.slt_tape_bytes(10, 16) % (28 * 10) == 0 and L.slt_tape_bytes(10, 16) <= 16 * 28 * 10
 
 
 def test_valu_calibrate_rejections(L):
