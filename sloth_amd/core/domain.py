@@ -98,8 +98,9 @@ class ScalarDomain:
         return np.broadcast_to(c.reshape(shape), (self.x_n, self.y_n, self.z_n))
 
     def _full(self, a):
-        return np.ascontiguousarray(np.broadcast_to(a, (self.x_n, self.y_n, self.z_n)),
-                                    dtype=np.float32)
+        # a fresh, writable C-order float32 grid (a broadcast view is read-only)
+        return np.array(np.broadcast_to(a, (self.x_n, self.y_n, self.z_n)), dtype=np.float32,
+                        order="C", copy=True)
 
     def test_slab(self):
         """ne_0 (1 + s x / L_x): a linear gradient along x (domain.py:580-603; s = 1,

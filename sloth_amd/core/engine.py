@@ -55,7 +55,10 @@ def to_device(a, device, stream=None) -> torch.Tensor:
     for everything queued on the stream before it, which stalls a host that is
     feeding several streams.  Larger arrays take the ordinary (blocking) copy.
     """
-    t = torch.from_numpy(np.ascontiguousarray(a))
+    arr = np.ascontiguousarray(a)
+    if not arr.flags.writeable:     # torch wraps only writable arrays
+        arr = arr.copy()
+    t = torch.from_numpy(arr)
     with torch.cuda.stream(stream):
         if t.numel() * t.element_size() <= _STAGED_MAX_BYTES:
             return t.pin_memory().to(device, non_blocking=True)

@@ -232,5 +232,5 @@ def test_half_spectrum_noise_is_hermitian_with_flat_power(n):
     for kz in planes:
         assert abs(power[kz] / power[1:n // 2].mean() - 1.0) < 6.0 / n
     # the real inverse transform of this spectrum is the full complex one's real part
-    full = np.fft.irfftn(F, s=(n, n, n))
+    full = np.fft.irfftn(F, s=(n, n, n), axes=(0, 1, 2))
     assert np.isfinite(full).all()

@@ -160,6 +160,9 @@ def test_scalar_domain_accepts_only_the_reference_keywords():
         ScalarDomain(1e-3, [8, 8])
     d = ScalarDomain(2e-3, [6, 7, 8])
     assert d.ne.shape == (6, 7, 8) and not d.ne.any()          # no ne_type: empty domain
+    for kind in ("test_slab", "test_linear_cos", "test_exponential_cos"):
+        ne = ScalarDomain(2e-3, [6, 7, 8], ne_type=kind).ne
+        assert ne.flags.writeable and ne.flags.c_contiguous and ne.dtype == np.float32
 
 
 def test_scalar_domain_generated_profiles():
