@@ -44,10 +44,10 @@ HBM_PEAK_GBS = 8000.0
 # from per-lane registers and L2/MALL, not HBM (reported, not a roof)
 BYTES_PER_STEP = 4 * 8 * 4 * 4
 # compulsory HBM bytes of one trace launch: the packed grid (16 B per voxel) once, and
-# per ray its start state (24 B) and order entry (4 B) read, final state (24 B), tau
-# (8 B), path (8 B) and step count (4 B) written
+# per ray its start state (24 B, already gathered into processing order) read, final
+# state (24 B), tau (8 B), path (8 B) and step count (4 B) written
 VOXEL_BYTES = 16
-RAY_IO_BYTES = 24 + 4 + 24 + 8 + 8 + 4
+RAY_IO_BYTES = 24 + 24 + 8 + 8 + 4
 
 
 def log(*a):
@@ -369,13 +369,18 @@ def main():
     stream = torch.cuda.current_stream()
 
     def step(events=None):
+        # Morton order, start states gathered into it, the trace over the gathered rays
+        # (coalesced state reads and writes), then the Jones epilogue back to the
+        # caller's ray order and the detector histogram
         order = engine.ray_order(grid, s0)
+        s0_sorted = engine.gather_rays(s0, order)
         if events is not None:
             events[0].record(stream)
-        res = engine.trace(grid, s0, save_times, save_all=False, want_path=True, order=order)
+        res = engine.trace(grid, s0_sorted, save_times, save_all=False, want_path=True,
+                           sort=False)
         if events is not None:
             events[1].record(stream)
-        rf, amp = jones_device(res.states[0], "z", res.tau[0])
+        rf, amp = jones_device(res.states[0], "z", res.tau[0], order=order)
         counts, wsum = histogram2d_device(rf[0], rf[2], edges, edges, weights=amp, order=order)
         return res, counts, wsum
 

@@ -152,6 +152,17 @@ int slt_jones(const float* state, int64_t ld, int64_t n, int32_t probing_axis, f
               const double* tau, double* amp, slt_stream_t stream);
 
 /*
+ * slt_jones for a state in processing order: state row t (and tau[t]) belongs to
+ * the caller's ray order[t], whose rf column and amp entry it writes, so rf and amp
+ * come out in the caller's order.  order must be a permutation of [0, n); an
+ * entry outside that range is skipped.  No reference counterpart (the inverse of
+ * slt_gather_rays; same formulas as slt_jones).
+ */
+int slt_jones_ordered(const float* state, int64_t ld, int64_t n, int32_t probing_axis,
+                      const int32_t* order, float* rf, const double* tau, double* amp,
+                      slt_stream_t stream);
+
+/*
  * k3: detector histogram.  Replaces numpy.histogram2d inside
  * Diagnostic.histogram (src/processing/diagnostics.py:243-276) including its
  * H.T transpose: counts/wsum are (nyb, nxb) row-major.  Bin b holds
@@ -221,6 +232,18 @@ int slt_rtm(const void* rays, int32_t rays_f64, int64_t ld, int64_t n, double po
  * input order, src/core/propagator.py:792).
  */
 int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int32_t* keys,
+                    slt_stream_t stream);
+
+/*
+ * Ray gather into processing order: s0_sorted[c][t] = s0[c][order[t]] for the six
+ * rows of a 6 x n fp32 start array (row stride n; s0 and s0_sorted must not be the
+ * same buffer).  Tracing the gathered rays with order = NULL reads and writes the
+ * state coalesced; slt_jones_ordered then puts the results back in the caller's
+ * order.  An order entry outside [0, n) gives a NaN row.  No reference
+ * counterpart (ray ordering is this library's; the reference vmaps rays in input
+ * order, src/core/propagator.py:792).
+ */
+int slt_gather_rays(const float* s0, int64_t n, const int32_t* order, float* s0_sorted,
                     slt_stream_t stream);
 
 /*

@@ -265,6 +265,9 @@ class TraceResult:
     order: torch.Tensor | None     # processing order used (Morton sort) or None
     nsub: np.ndarray
     h: np.ndarray
+    # True: columns are in processing order (column t is the caller's ray order[t]);
+    # False: columns are in the caller's order
+    permuted: bool = False
 
 
 def ray_order(grid: PackedGrid, s0: torch.Tensor, stream=None) -> torch.Tensor:
@@ -279,6 +282,18 @@ def ray_order(grid: PackedGrid, s0: torch.Tensor, stream=None) -> torch.Tensor:
         return torch.sort(keys, stable=True).indices.to(torch.int32)
 
 
+def gather_rays(s0: torch.Tensor, order: torch.Tensor, stream=None) -> torch.Tensor:
+    """(6, N) start states in processing order: column t is the caller's ray order[t]."""
+    n = s0.shape[1]
+    if order.dtype != torch.int32 or order.shape != (n,) or not order.is_contiguous():
+        raise AssertionError("order must be a contiguous int32 permutation of the rays")
+    with torch.cuda.stream(stream):
+        out = torch.empty_like(s0)
+        N.check(N.lib().slt_gather_rays(s0.data_ptr(), n, order.data_ptr(), out.data_ptr(),
+                                        N.stream_handle(stream)))
+    return out
+
+
 def as_rays(s0, device) -> torch.Tensor:
     """(6, N) float32 contiguous device tensor from any array-like."""
     t = device_f32(s0, device)
@@ -289,10 +304,15 @@ def as_rays(s0, device) -> torch.Tensor:
 
 def trace(grid: PackedGrid, s0, save_times, *, step_fraction: float = 0.5,
           save_all: bool = True, want_path: bool = True, sort: bool = True,
-          order: torch.Tensor | None = None, stream=None) -> TraceResult:
+          order: torch.Tensor | None = None, gather: bool = False,
+          stream=None) -> TraceResult:
     """Integrate rays through ``grid`` with fixed-step RK4, saving at ``save_times``.
 
     ``save_times`` are real times [s] starting at 0 (the initial state).
+    ``gather=True`` (with a processing order) gathers the start states into that
+    order first and traces them in place, so that every state read and write is
+    coalesced; the result's columns then stay in processing order
+    (``permuted=True``), for ``jones_device(..., order=result.order)`` to put back.
     """
     rays = as_rays(s0, grid.device)
     n = rays.shape[1]
@@ -311,12 +331,15 @@ def trace(grid: PackedGrid, s0, save_times, *, step_fraction: float = 0.5,
         return TraceResult(states, tau, path, steps, total, None, nsub, h)
     if order is None and sort:
         order = ray_order(grid, rays, stream)
+    permuted = gather and order is not None
+    if permuted:
+        rays = gather_rays(rays, order, stream)
     seg_nsub = const_device(nsub, dev, stream)
     seg_h = const_device(h.astype(np.float32), dev, stream)
     a = N.TraceArgs()
     a.n_rays = n
     a.s0 = rays.data_ptr()
-    a.order = N.ptr(order)
+    a.order = None if permuted else N.ptr(order)
     a.n_seg = nseg
     a.save_every = 1 if save_all else 0
     a.seg_nsub = seg_nsub.data_ptr()
@@ -328,4 +351,4 @@ def trace(grid: PackedGrid, s0, save_times, *, step_fraction: float = 0.5,
     a.total_steps = total.data_ptr()
     N.check(N.lib().slt_trace(grid.voxels.data_ptr(), ctypes.byref(grid.desc), ctypes.byref(a),
                               N.stream_handle(stream)))
-    return TraceResult(states, tau, path, steps, total, order, nsub, h)
+    return TraceResult(states, tau, path, steps, total, order, nsub, h, permuted)

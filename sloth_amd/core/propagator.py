@@ -137,8 +137,10 @@ def solve(beam, ScalarDomain, probing_depth, *, parallelise=True, jitted=True,
     save_times = np.linspace(0.0, t_end, n_save)
     torch.cuda.synchronize()
     start = time.time()
+    # raw results keep the caller's ray order; otherwise the rays are traced in
+    # processing order and the Jones epilogue puts them back
     res = engine.trace(grid, rays, save_times, step_fraction=step_fraction,
-                       save_all=n_save > 2, want_path=False)
+                       save_all=n_save > 2, want_path=False, gather=not return_raw_results)
     torch.cuda.synchronize()
     duration = np.float64(time.time() - start)
     if verbose:
@@ -146,13 +148,15 @@ def solve(beam, ScalarDomain, probing_depth, *, parallelise=True, jitted=True,
     if return_raw_results:
         return res, None, duration
     pd = ScalarDomain.probing_direction
+    back = res.order if res.permuted else None
     if n_save == 2:
-        rf, amp = jones_device(res.states[-1], pd, res.tau[-1] if inv_brems else None)
+        rf, amp = jones_device(res.states[-1], pd, res.tau[-1] if inv_brems else None,
+                               order=back)
         rf = rf.cpu().numpy()
         if inv_brems:
             return rf, amp.cpu().numpy(), duration
         return rf, None, duration
-    rfs = [jones_device(res.states[j], pd)[0] for j in range(res.states.shape[0])]
+    rfs = [jones_device(res.states[j], pd, order=back)[0] for j in range(res.states.shape[0])]
     return torch.stack(rfs).cpu().numpy(), None, duration
 
 
@@ -206,14 +210,16 @@ def trace_and_save_depths(beam, ScalarDomain, step, depth_max, output_path, *,
     n = rays.shape[1]
     if n_saves >= 2:
         res = engine.trace(grid, rays, depth_saves / C_LIGHT, step_fraction=step_fraction,
-                           save_all=True, want_path=False)
+                           save_all=True, want_path=False, gather=True)
         states, taus = res.states, res.tau
+        back = res.order if res.permuted else None
     else:
         states = rays.unsqueeze(0)
         taus = torch.zeros((1, n), dtype=torch.float64, device=dev) if inv_brems else None
+        back = None
     jvec_list, amp_list = [], ([] if inv_brems else None)
     for j in range(n_saves):
-        rf, amp = jones_device(states[j], pd, taus[j] if inv_brems else None)
+        rf, amp = jones_device(states[j], pd, taus[j] if inv_brems else None, order=back)
         jvec_list.append(rf.cpu().numpy()[comp, :])
         if inv_brems:
             amp_list.append(amp.cpu().numpy())

@@ -55,6 +55,20 @@ __global__ void __launch_bounds__(256) morton_kernel(const float* __restrict__ s
     keys[i] = (int32_t)(spread15(qa) | (spread15(qb) << 1));
 }
 
+// Row t of the output takes the caller's ray order[t]: one scattered read per
+// component, coalesced writes.  An entry outside [0, n) gives a NaN row.
+__global__ void __launch_bounds__(256) gather_rays_kernel(const float* __restrict__ s0,
+                                                          int64_t n,
+                                                          const int32_t* __restrict__ order,
+                                                          float* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t i = order[t];
+    const bool ok = i >= 0 && i < n;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) out[c * n + t] = ok ? s0[c * n + i] : __builtin_nanf("");
+}
+
 }  // namespace slt
 
 using namespace slt;
@@ -141,6 +155,21 @@ int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int32_t*
                        (hipStream_t)stream, s0, n, ax[0], ax[1], g->lo[ax[0]], g->hi[ax[0]],
                        g->lo[ax[1]], g->hi[ax[1]], keys);
     return check_launch("slt_morton_keys");
+}
+
+int slt_gather_rays(const float* s0, int64_t n, const int32_t* order, float* s0_sorted,
+                    slt_stream_t stream) {
+    clear_error();
+    if (n < 0 || (n > 0 && (!s0 || !order || !s0_sorted)) ||
+        (n > 0 && (const void*)s0 == (const void*)s0_sorted)) {
+        set_error("slt_gather_rays: invalid arguments");
+        return SLT_ERR_INVALID;
+    }
+    if (n == 0) return SLT_OK;
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(gather_rays_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       (hipStream_t)stream, s0, n, order, s0_sorted);
+    return check_launch("slt_gather_rays");
 }
 
 }  // extern "C"

@@ -15,12 +15,19 @@
 
 namespace slt {
 
+// Row i of the state (and tau[i]) goes to output column o = order[i] when ORDERED
+// (the state is in processing order, the outputs in the caller's), else o = i.
+template <bool ORDERED>
 __global__ void __launch_bounds__(256) jones_kernel(const float* __restrict__ st, int64_t ld,
-                                                    int64_t n, int axis, float* __restrict__ rf,
+                                                    int64_t n, int axis,
+                                                    const int32_t* __restrict__ order,
+                                                    float* __restrict__ rf,
                                                     const double* __restrict__ tau,
                                                     double* __restrict__ amp) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int64_t o = ORDERED ? (int64_t)order[i] : i;
+    if (ORDERED && (o < 0 || o >= n)) return;  // not a permutation entry
     const float x = st[i], y = st[ld + i], z = st[2 * ld + i];
     const float vx = st[3 * ld + i], vy = st[4 * ld + i], vz = st[5 * ld + i];
     float p1, a1, p2, a2;
@@ -31,11 +38,11 @@ __global__ void __launch_bounds__(256) jones_kernel(const float* __restrict__ st
     } else {
         p1 = x; a1 = atanf(vx / vz); p2 = y; a2 = atanf(vy / vz);
     }
-    rf[i] = p1;
-    rf[n + i] = a1;
-    rf[2 * n + i] = p2;
-    rf[3 * n + i] = a2;
-    if (amp && tau) amp[i] = exp(-tau[i]);
+    rf[o] = p1;
+    rf[n + o] = a1;
+    rf[2 * n + o] = p2;
+    rf[3 * n + o] = a2;
+    if (amp && tau) amp[o] = exp(-tau[i]);
 }
 
 // numpy bin index in [0, nb+1]: 0 = below, nb+1 = above (dropped), else bin+1.
@@ -130,9 +137,25 @@ extern "C" int slt_jones(const float* state, int64_t ld, int64_t n, int32_t prob
     }
     if (n == 0) return SLT_OK;
     const int64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(jones_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
-                       state, ld, n, probing_axis, rf, tau, amp);
+    hipLaunchKernelGGL(jones_kernel<false>, dim3((unsigned)blocks), dim3(256), 0,
+                       (hipStream_t)stream, state, ld, n, probing_axis, nullptr, rf, tau, amp);
     return check_launch("slt_jones");
+}
+
+extern "C" int slt_jones_ordered(const float* state, int64_t ld, int64_t n,
+                                 int32_t probing_axis, const int32_t* order, float* rf,
+                                 const double* tau, double* amp, slt_stream_t stream) {
+    clear_error();
+    if (n < 0 || probing_axis < 0 || probing_axis > 2 ||
+        (n > 0 && (!state || !rf || !order)) || ld < n) {
+        set_error("slt_jones_ordered: invalid arguments");
+        return SLT_ERR_INVALID;
+    }
+    if (n == 0) return SLT_OK;
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(jones_kernel<true>, dim3((unsigned)blocks), dim3(256), 0,
+                       (hipStream_t)stream, state, ld, n, probing_axis, order, rf, tau, amp);
+    return check_launch("slt_jones_ordered");
 }
 
 extern "C" int slt_histogram2d(const void* x, int64_t x_stride, const void* y,

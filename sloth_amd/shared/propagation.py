@@ -10,8 +10,13 @@ from .. import _native as N
 from ..core.engine import axis_index, device_f32
 
 
-def jones_device(state: torch.Tensor, probing_direction: str = "z", tau: torch.Tensor | None = None):
-    """(6, N) (or (>=6, N)) float32 device state -> ((4, N) float32, amp (N,) float64 | None)."""
+def jones_device(state: torch.Tensor, probing_direction: str = "z", tau: torch.Tensor | None = None,
+                 order: torch.Tensor | None = None):
+    """(6, N) (or (>=6, N)) float32 device state -> ((4, N) float32, amp (N,) float64 | None).
+
+    With ``order`` the state (and tau) are in processing order, column t being the
+    caller's ray order[t], and rf / amp come out in the caller's order.
+    """
     if state.dim() != 2 or state.shape[0] < 6:
         raise AssertionError("expected a (6, N) ray state")
     st = state if state.is_contiguous() else state.contiguous()
@@ -21,8 +26,16 @@ def jones_device(state: torch.Tensor, probing_direction: str = "z", tau: torch.T
     if tau is not None:
         tau = tau.to(torch.float64).contiguous()
         amp = torch.empty(n, dtype=torch.float64, device=st.device)
-    N.check(N.lib().slt_jones(st.data_ptr(), n, n, axis_index(probing_direction), rf.data_ptr(),
-                              N.ptr(tau), N.ptr(amp), N.stream_handle()))
+    axis = axis_index(probing_direction)
+    if order is None:
+        N.check(N.lib().slt_jones(st.data_ptr(), n, n, axis, rf.data_ptr(), N.ptr(tau),
+                                  N.ptr(amp), N.stream_handle()))
+    else:
+        if order.dtype != torch.int32 or order.shape != (n,) or not order.is_contiguous():
+            raise AssertionError("order must be a contiguous int32 permutation of the rays")
+        N.check(N.lib().slt_jones_ordered(st.data_ptr(), n, n, axis, order.data_ptr(),
+                                          rf.data_ptr(), N.ptr(tau), N.ptr(amp),
+                                          N.stream_handle()))
     return rf, amp
 
 

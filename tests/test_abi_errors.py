@@ -148,6 +148,30 @@ def test_jones_histogram_morton_rejections(L):
     _expect(L, L.slt_morton_keys(None, 0, ctypes.byref(g), None, None), N.SLT_OK, None)
 
 
+def test_gather_and_ordered_jones_rejections(L):
+    other = FAKE + 4096
+    _expect(L, L.slt_gather_rays(FAKE, -1, FAKE, other, None), N.SLT_ERR_INVALID,
+            "slt_gather_rays")
+    for s0, order, out in ((None, FAKE, other), (FAKE, None, other), (FAKE, other, None)):
+        _expect(L, L.slt_gather_rays(s0, 10, order, out, None), N.SLT_ERR_INVALID,
+                "slt_gather_rays")
+    # gathering in place would overwrite rays before they are read
+    _expect(L, L.slt_gather_rays(FAKE, 10, other, FAKE, None), N.SLT_ERR_INVALID,
+            "slt_gather_rays")
+    _expect(L, L.slt_gather_rays(None, 0, None, None, None), N.SLT_OK, None)
+
+    _expect(L, L.slt_jones_ordered(FAKE, 10, 10, 2, None, FAKE, None, None, None),
+            N.SLT_ERR_INVALID, "slt_jones_ordered")      # order is required
+    _expect(L, L.slt_jones_ordered(FAKE, 10, 10, 3, FAKE, FAKE, None, None, None),
+            N.SLT_ERR_INVALID, "slt_jones_ordered")
+    _expect(L, L.slt_jones_ordered(FAKE, 5, 10, 2, FAKE, FAKE, None, None, None),
+            N.SLT_ERR_INVALID, "slt_jones_ordered")
+    _expect(L, L.slt_jones_ordered(FAKE, 10, -1, 2, FAKE, FAKE, None, None, None),
+            N.SLT_ERR_INVALID, "slt_jones_ordered")
+    _expect(L, L.slt_jones_ordered(None, 0, 0, 2, None, None, None, None, None), N.SLT_OK,
+            None)
+
+
 def test_rtm_and_beam_rejections(L):
     ops = (N.Optic * (N.SLT_MAX_OPTICS + 1))()
     _expect(L, L.slt_rtm(FAKE, 0, 10, 10, 1.0, ops, N.SLT_MAX_OPTICS + 1, FAKE, 10, None),

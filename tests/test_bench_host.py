@@ -55,8 +55,8 @@ def test_hbm_roofline_arithmetic():
     n_vox, n_rays = 512 ** 3, 10_000_000
     r = bench.hbm_roofline(traffic / 8.0e12, n_vox, n_rays)   # duration at exactly 8 TB/s
     assert r["frac"] == pytest.approx(1.0, abs=1e-4) and r["unit"] == "GB/s"
-    # compulsory: 16 B per voxel + 72 B of per-ray I/O
-    assert r["compulsory_bytes"] == 16 * n_vox + 72 * n_rays
+    # compulsory: 16 B per voxel + 68 B of per-ray I/O
+    assert r["compulsory_bytes"] == 16 * n_vox + 68 * n_rays
     assert r["traffic_over_compulsory"] == pytest.approx(traffic / r["compulsory_bytes"],
                                                          rel=1e-3)
     json.dumps(r)

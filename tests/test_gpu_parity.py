@@ -247,6 +247,59 @@ def test_trace_order_invariance_and_determinism():
         np.testing.assert_array_equal(a[k], c[k])
 
 
+def test_gathered_trace_equals_permuted_io():
+    """Rays gathered into Morton order and traced in place give, column for column, the
+    states, tau, path and step counts of the trace that reads and writes through the
+    order array; the ordered Jones epilogue puts rf and amp back in the caller's order
+    bit for bit; solve returns the same rf either way."""
+    import torch
+
+    from sloth_amd.core import engine
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.core.domain import ScalarDomain
+    from sloth_amd.core.propagator import solve
+    from sloth_amd.fields import kolmogorov_field
+    from sloth_amd.shared.propagation import jones_device
+
+    L = 6e-3
+    ne, coords = kolmogorov_field(64, L, seed=9, n0=4e24, frac=0.3, l_min=0.4e-3)
+    s0 = Beam(60000, 2.5e-3, 2e-4, L / 2, beam_type="square", seeded=True).s0
+    grid = engine.PackedGrid(ne, *coords, 2 * np.pi * C / 532e-9, inv_brems=True, Te=20.0,
+                             Z=1.0)
+    t = np.linspace(0, 2.5 * L / C, 4)
+    a = engine.trace(grid, s0, t, save_all=True, want_path=True)
+    b = engine.trace(grid, s0, t, save_all=True, want_path=True, gather=True)
+    assert not a.permuted and b.permuted and torch.equal(a.order, b.order)
+    o = a.order.long()
+    assert torch.equal(torch.sort(o).values, torch.arange(o.numel(), device=o.device))
+    assert torch.equal(b.states, a.states[:, :, o])
+    assert torch.equal(b.tau, a.tau[:, o])
+    assert torch.equal(b.path, a.path[o]) and torch.equal(b.steps, a.steps[o])
+    assert torch.equal(a.total_steps, b.total_steps)
+    for j in range(t.size):
+        rf_a, amp_a = jones_device(a.states[j], "z", a.tau[j])
+        rf_b, amp_b = jones_device(b.states[j], "z", b.tau[j], order=b.order)
+        assert torch.equal(rf_a, rf_b) and torch.equal(amp_a, amp_b)
+    # an order entry outside [0, n) gathers a NaN row and leaves the others alone
+    bad = b.order.clone()
+    bad[3], bad[7] = -1, o.numel()
+    rays = engine.as_rays(s0, grid.device)
+    g = engine.gather_rays(rays, bad)
+    assert torch.isnan(g[:, [3, 7]]).all()
+    keep = torch.ones(o.numel(), dtype=torch.bool, device=o.device)
+    keep[[3, 7]] = False
+    assert torch.equal(g[:, keep], rays[:, o[keep]])
+    # solve: the gathered path (default) against the raw caller-order states
+    dom = ScalarDomain(lengths=np.array([L, L, L]), dims=np.array(ne.shape), ne=ne,
+                       probing_direction="z", inv_brems=True, Te=20.0, Z=1.0)
+    rf, amp, _ = solve(s0, dom, L, lwl=532e-9, verbose=False)
+    raw, _, _ = solve(s0, dom, L, lwl=532e-9, verbose=False, return_raw_results=True)
+    assert not raw.permuted
+    rf_raw, amp_raw = jones_device(raw.states[-1], "z", raw.tau[-1])
+    np.testing.assert_array_equal(rf, rf_raw.cpu().numpy())
+    np.testing.assert_array_equal(amp, amp_raw.cpu().numpy())
+
+
 def test_vacuum_is_exact():
     ne = np.zeros((12, 12, 12), np.float32)
     c = np.float32(np.linspace(-2e-3, 2e-3, 12))
