@@ -368,21 +368,25 @@ def main():
     edges = np.linspace(-L / 2, L / 2, 257)
     stream = torch.cuda.current_stream()
 
+    work = engine.ray_work(args.rays, dev)
+
     def step(events=None):
         # Morton order, start states gathered into it, the trace over the gathered rays
-        # (coalesced state reads and writes), then the Jones epilogue back to the
-        # caller's ray order and the detector histogram
+        # (coalesced state reads and writes), the Jones epilogue and the detector
+        # histogram in that order, then rf and amp back in the caller's ray order
         order = engine.ray_order(grid, s0)
-        s0_sorted = engine.gather_rays(s0, order)
+        s0_sorted = engine.gather_rays(s0, order, work=work)
         if events is not None:
             events[0].record(stream)
         res = engine.trace(grid, s0_sorted, save_times, save_all=False, want_path=True,
                            sort=False)
         if events is not None:
             events[1].record(stream)
-        rf, amp = jones_device(res.states[0], "z", res.tau[0], order=order)
-        counts, wsum = histogram2d_device(rf[0], rf[2], edges, edges, weights=amp, order=order)
-        return res, counts, wsum
+        rf_sorted, amp_sorted = jones_device(res.states[0], "z", res.tau[0])
+        counts, wsum = histogram2d_device(rf_sorted[0], rf_sorted[2], edges, edges,
+                                          weights=amp_sorted)
+        rf, amp = engine.unpermute_rays(rf_sorted, amp_sorted, order, work=work)
+        return res, counts, wsum, rf, amp
 
     log("[bench] warmup ...")
     for _ in range(args.warmup):
@@ -398,7 +402,7 @@ def main():
         results.append(step(evs[k]))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    res, counts, _ = results[-1]
+    res, counts = results[-1][0], results[-1][1]
     per_pass = [int(r[0].total_steps.item()) for r in results]
     total_steps = per_pass[0]
     assert all(p == total_steps for p in per_pass), per_pass

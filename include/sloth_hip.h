@@ -152,17 +152,6 @@ int slt_jones(const float* state, int64_t ld, int64_t n, int32_t probing_axis, f
               const double* tau, double* amp, slt_stream_t stream);
 
 /*
- * slt_jones for a state in processing order: state row t (and tau[t]) belongs to
- * the caller's ray order[t], whose rf column and amp entry it writes, so rf and amp
- * come out in the caller's order.  order must be a permutation of [0, n); an
- * entry outside that range is skipped.  No reference counterpart (the inverse of
- * slt_gather_rays; same formulas as slt_jones).
- */
-int slt_jones_ordered(const float* state, int64_t ld, int64_t n, int32_t probing_axis,
-                      const int32_t* order, float* rf, const double* tau, double* amp,
-                      slt_stream_t stream);
-
-/*
  * k3: detector histogram.  Replaces numpy.histogram2d inside
  * Diagnostic.histogram (src/processing/diagnostics.py:243-276) including its
  * H.T transpose: counts/wsum are (nyb, nxb) row-major.  Bin b holds
@@ -174,10 +163,12 @@ int slt_jones_ordered(const float* state, int64_t ld, int64_t n, int32_t probing
  * counts (uint32) and/or wsum (fp64, needs weights) are accumulated into, so
  * the caller zeroes them first.  mask: optional device uint8 per ray (0 = skip).
  * order: optional device int32 permutation of [0, n) (the Morton order the
- * rays were traced in); rays are then visited as order[0..n) and a
- * wavefront's entries for one cell are combined into one atomic.  Counts do
- * not depend on it; wsum only through fp64 summation order.  Entries outside
- * [0, n) are skipped.  nxb * nyb must fit in int32.
+ * rays were traced in); rays are then visited as order[0..n), else in index
+ * order.  A wavefront combines its entries for each of its first few distinct
+ * cells into one atomic, so rays visited in a spatially coherent order (Morton
+ * order, or results already in that order) make few atomics on one address.
+ * Counts do not depend on the visiting order; wsum only through fp64 summation
+ * order.  Order entries outside [0, n) are skipped.  nxb * nyb must fit in int32.
  */
 int slt_histogram2d(const void* x, int64_t x_stride, const void* y, int64_t y_stride,
                     int32_t is_f64, int64_t n, const int32_t* order, const uint8_t* mask,
@@ -235,16 +226,31 @@ int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int32_t*
                     slt_stream_t stream);
 
 /*
- * Ray gather into processing order: s0_sorted[c][t] = s0[c][order[t]] for the six
- * rows of a 6 x n fp32 start array (row stride n; s0 and s0_sorted must not be the
- * same buffer).  Tracing the gathered rays with order = NULL reads and writes the
- * state coalesced; slt_jones_ordered then puts the results back in the caller's
- * order.  An order entry outside [0, n) gives a NaN row.  No reference
- * counterpart (ray ordering is this library's; the reference vmaps rays in input
- * order, src/core/propagator.py:792).
+ * Ray permutation between the caller's order and a processing order (order[t] =
+ * the caller's index of the ray processed t-th).  No reference counterpart (ray
+ * ordering is this library's; the reference vmaps rays in input order,
+ * src/core/propagator.py:792).  Both calls go through `work`, a caller-owned device
+ * buffer of slt_ray_work_bytes(n) bytes (32 B per ray) that must not overlap the
+ * data: a scattered 4-byte access moves a whole cache line, so each ray is moved as
+ * one 32-byte record, with a coalesced transpose between rows and records.
+ *
+ * slt_gather_rays: s0_sorted[c][t] = s0[c][order[t]] for the six rows of a 6 x n
+ * fp32 start array (row stride n); s0_sorted may be s0 itself.  An order entry
+ * outside [0, n) gives a NaN row.  Tracing the gathered rays with order = NULL
+ * reads and writes the state coalesced.
+ *
+ * slt_unpermute_rays: the inverse, for results in processing order:
+ * dst[c][order[t]] = src[c][t] for `rows` fp32 rows (row stride n) and
+ * dst64[order[t]] = src64[t] for an optional fp64 row; rows + 2 (with the fp64 row)
+ * <= 8; dst may be src.  order must be a permutation of [0, n) (entries outside it
+ * are skipped, leaving their columns undefined).
  */
+int64_t slt_ray_work_bytes(int64_t n);
 int slt_gather_rays(const float* s0, int64_t n, const int32_t* order, float* s0_sorted,
-                    slt_stream_t stream);
+                    void* work, slt_stream_t stream);
+int slt_unpermute_rays(const float* src, int32_t rows, const double* src64, int64_t n,
+                       const int32_t* order, float* dst, double* dst64, void* work,
+                       slt_stream_t stream);
 
 /*
  * k6: probe beam made on the device (opt-in; no reference counterpart: the

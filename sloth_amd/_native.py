@@ -23,7 +23,7 @@ EXPORTED = (
     "slt_ray_state_bytes", "slt_ray_state_init", "slt_job_table_prepare", "slt_advance",
     "slt_checkpoint_bytes", "slt_tape_bytes", "slt_forward_ckpt", "slt_adjoint",
     "slt_adjoint_unpack", "slt_rtm", "slt_beam", "slt_adjoint_noalloc", "slt_valu_calibrate",
-    "slt_gather_rays", "slt_jones_ordered",
+    "slt_ray_work_bytes", "slt_gather_rays", "slt_unpermute_rays",
 )
 
 SLT_MAX_JOBS = 64
@@ -163,8 +163,10 @@ def lib():
     L.slt_kappa.argtypes = [vp, i64, vp, i64, vp, i64, dbl, dbl, dbl, dbl, dbl, vp, vp]
     L.slt_trace.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(TraceArgs), vp]
     L.slt_jones.argtypes = [vp, i64, i64, i32, vp, vp, vp, vp]
-    L.slt_jones_ordered.argtypes = [vp, i64, i64, i32, vp, vp, vp, vp, vp]
-    L.slt_gather_rays.argtypes = [vp, i64, vp, vp, vp]
+    L.slt_ray_work_bytes.argtypes = [i64]
+    L.slt_ray_work_bytes.restype = i64
+    L.slt_gather_rays.argtypes = [vp, i64, vp, vp, vp, vp]
+    L.slt_unpermute_rays.argtypes = [vp, i32, vp, i64, vp, vp, vp, vp, vp]
     L.slt_histogram2d.argtypes = [vp, i64, vp, i64, i32, i64, vp, vp, vp, i32, vp, i32, vp, vp, vp,
                                   vp]
     L.slt_morton_keys.argtypes = [vp, i64, ctypes.POINTER(GridDesc), vp, vp]

@@ -282,16 +282,57 @@ def ray_order(grid: PackedGrid, s0: torch.Tensor, stream=None) -> torch.Tensor:
         return torch.sort(keys, stable=True).indices.to(torch.int32)
 
 
-def gather_rays(s0: torch.Tensor, order: torch.Tensor, stream=None) -> torch.Tensor:
-    """(6, N) start states in processing order: column t is the caller's ray order[t]."""
-    n = s0.shape[1]
+def _check_order(order: torch.Tensor, n: int) -> None:
     if order.dtype != torch.int32 or order.shape != (n,) or not order.is_contiguous():
         raise AssertionError("order must be a contiguous int32 permutation of the rays")
+
+
+def ray_work(n: int, device) -> torch.Tensor:
+    """Device workspace for gather_rays / unpermute_rays over n rays (32 B per ray)."""
+    return torch.empty(max(int(N.lib().slt_ray_work_bytes(n)), 1), dtype=torch.uint8,
+                       device=device)
+
+
+def gather_rays(s0: torch.Tensor, order: torch.Tensor, stream=None,
+                work: torch.Tensor | None = None, out: torch.Tensor | None = None) -> torch.Tensor:
+    """(6, N) start states in processing order: column t is the caller's ray order[t]."""
+    n = s0.shape[1]
+    _check_order(order, n)
     with torch.cuda.stream(stream):
-        out = torch.empty_like(s0)
+        if work is None:
+            work = ray_work(n, s0.device)
+        if out is None:
+            out = torch.empty_like(s0)
         N.check(N.lib().slt_gather_rays(s0.data_ptr(), n, order.data_ptr(), out.data_ptr(),
-                                        N.stream_handle(stream)))
+                                        work.data_ptr(), N.stream_handle(stream)))
     return out
+
+
+def unpermute_rays(src: torch.Tensor | None, src64: torch.Tensor | None, order: torch.Tensor,
+                   stream=None, work: torch.Tensor | None = None):
+    """Results in processing order back to the caller's: returns (dst, dst64) with
+    dst[:, order[t]] = src[:, t] for the fp32 rows of `src` (r, N) and
+    dst64[order[t]] = src64[t] for an optional fp64 row."""
+    n = order.numel()
+    _check_order(order, n)
+    rows = 0 if src is None else src.shape[0]
+    if src is not None:
+        if src.dtype != torch.float32 or src.dim() != 2 or src.shape[1] != n:
+            raise AssertionError("src must be an (r, N) float32 tensor")
+        src = src.contiguous()
+    if src64 is not None:
+        if src64.dtype != torch.float64 or src64.shape != (n,):
+            raise AssertionError("src64 must be an (N,) float64 tensor")
+        src64 = src64.contiguous()
+    with torch.cuda.stream(stream):
+        if work is None:
+            work = ray_work(n, order.device)
+        dst = torch.empty_like(src) if src is not None else None
+        dst64 = torch.empty_like(src64) if src64 is not None else None
+        N.check(N.lib().slt_unpermute_rays(N.ptr(src), rows, N.ptr(src64), n, order.data_ptr(),
+                                           N.ptr(dst), N.ptr(dst64), work.data_ptr(),
+                                           N.stream_handle(stream)))
+    return dst, dst64
 
 
 def as_rays(s0, device) -> torch.Tensor:

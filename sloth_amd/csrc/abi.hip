@@ -55,18 +55,92 @@ __global__ void __launch_bounds__(256) morton_kernel(const float* __restrict__ s
     keys[i] = (int32_t)(spread15(qa) | (spread15(qb) << 1));
 }
 
-// Row t of the output takes the caller's ray order[t]: one scattered read per
-// component, coalesced writes.  An entry outside [0, n) gives a NaN row.
-__global__ void __launch_bounds__(256) gather_rays_kernel(const float* __restrict__ s0,
-                                                          int64_t n,
-                                                          const int32_t* __restrict__ order,
-                                                          float* __restrict__ out) {
+// Per-ray 32-byte records for permuting rays.  A scattered 4-byte access moves
+// a whole 128-byte line, so permuting six SoA rows element by element moves six
+// lines per ray; a record moves one.  Each permutation is therefore a coalesced
+// transpose between the SoA rows and records in one order plus one record
+// access per ray at its position in the other order.
+struct __attribute__((aligned(32))) RayRec {
+    float v[8];
+};
+
+// rec[i] = the six start-state rows of ray i (caller order, coalesced).
+__global__ void __launch_bounds__(256) rays_to_rec_kernel(const float* __restrict__ s0, int64_t n,
+                                                          RayRec* __restrict__ rec) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    RayRec r;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) r.v[c] = s0[c * n + i];
+    r.v[6] = 0.f;
+    r.v[7] = 0.f;
+    rec[i] = r;
+}
+
+// Row t of the output takes the caller's ray order[t]; an entry outside [0, n)
+// gives a NaN row.
+__global__ void __launch_bounds__(256) gather_rec_kernel(const RayRec* __restrict__ rec,
+                                                         int64_t n,
+                                                         const int32_t* __restrict__ order,
+                                                         float* __restrict__ out) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int64_t i = order[t];
-    const bool ok = i >= 0 && i < n;
+    RayRec r;
+    if (i >= 0 && i < n) {
+        r = rec[i];
+    } else {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) out[c * n + t] = ok ? s0[c * n + i] : __builtin_nanf("");
+        for (int c = 0; c < 6; ++c) r.v[c] = __builtin_nanf("");
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) out[c * n + t] = r.v[c];
+}
+
+// Column t of `rows` fp32 rows (and of the fp64 row src64, in the two slots after
+// them) goes to record order[t]; entries outside [0, n) are skipped.
+__global__ void __launch_bounds__(256) scatter_rec_kernel(const float* __restrict__ src, int rows,
+                                                          const double* __restrict__ src64,
+                                                          int64_t n,
+                                                          const int32_t* __restrict__ order,
+                                                          RayRec* __restrict__ rec) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t o = order[t];
+    if (o < 0 || o >= n) return;
+    RayRec r;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) r.v[c] = c < rows ? src[c * n + t] : 0.f;
+    if (src64) {
+        const float2 b = __builtin_bit_cast(float2, src64[t]);
+#pragma unroll
+        for (int c = 0; c < 7; ++c)
+            if (c == rows) {
+                r.v[c] = b.x;
+                r.v[c + 1] = b.y;
+            }
+    }
+    rec[o] = r;
+}
+
+// The records back to rows (caller order, coalesced).
+__global__ void __launch_bounds__(256) rec_to_rows_kernel(const RayRec* __restrict__ rec,
+                                                          int rows, int64_t n,
+                                                          float* __restrict__ dst,
+                                                          double* __restrict__ dst64) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const RayRec r = rec[i];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (c < rows) dst[c * n + i] = r.v[c];
+    if (dst64) {
+        float2 b = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int c = 0; c < 7; ++c)
+            if (c == rows) b = make_float2(r.v[c], r.v[c + 1]);
+        dst64[i] = __builtin_bit_cast(double, b);
+    }
 }
 
 }  // namespace slt
@@ -157,19 +231,61 @@ int slt_morton_keys(const float* s0, int64_t n, const slt_grid_desc* g, int32_t*
     return check_launch("slt_morton_keys");
 }
 
+int64_t slt_ray_work_bytes(int64_t n) { return n > 0 ? n * (int64_t)sizeof(RayRec) : 0; }
+
+// [a, a + na) and [b, b + nb) share a byte
+static bool overlaps(const void* a, int64_t na, const void* b, int64_t nb) {
+    const char* pa = static_cast<const char*>(a);
+    const char* pb = static_cast<const char*>(b);
+    return pa < pb + nb && pb < pa + na;
+}
+
 int slt_gather_rays(const float* s0, int64_t n, const int32_t* order, float* s0_sorted,
-                    slt_stream_t stream) {
+                    void* work, slt_stream_t stream) {
     clear_error();
-    if (n < 0 || (n > 0 && (!s0 || !order || !s0_sorted)) ||
-        (n > 0 && (const void*)s0 == (const void*)s0_sorted)) {
+    if (n < 0 || (n > 0 && (!s0 || !order || !s0_sorted || !work))) {
         set_error("slt_gather_rays: invalid arguments");
         return SLT_ERR_INVALID;
     }
     if (n == 0) return SLT_OK;
-    const int64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(gather_rays_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                       (hipStream_t)stream, s0, n, order, s0_sorted);
+    const int64_t wb = slt_ray_work_bytes(n);
+    if (overlaps(work, wb, s0, 24 * n) || overlaps(work, wb, s0_sorted, 24 * n)) {
+        set_error("slt_gather_rays: work overlaps the rays");
+        return SLT_ERR_INVALID;
+    }
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    hipStream_t st = (hipStream_t)stream;
+    RayRec* rec = static_cast<RayRec*>(work);
+    hipLaunchKernelGGL(rays_to_rec_kernel, grid, block, 0, st, s0, n, rec);
+    hipLaunchKernelGGL(gather_rec_kernel, grid, block, 0, st, rec, n, order, s0_sorted);
     return check_launch("slt_gather_rays");
+}
+
+int slt_unpermute_rays(const float* src, int32_t rows, const double* src64, int64_t n,
+                       const int32_t* order, float* dst, double* dst64, void* work,
+                       slt_stream_t stream) {
+    clear_error();
+    if (n < 0 || rows < 0 || rows + (src64 ? 2 : 0) > 8 || (rows == 0 && !src64) ||
+        (!src64) != (!dst64) ||
+        (n > 0 && ((rows > 0 && (!src || !dst)) || !order || !work))) {
+        set_error("slt_unpermute_rays: invalid arguments");
+        return SLT_ERR_INVALID;
+    }
+    if (n == 0) return SLT_OK;
+    const int64_t wb = slt_ray_work_bytes(n);
+    if ((rows > 0 && (overlaps(work, wb, src, 4 * rows * n) ||
+                      overlaps(work, wb, dst, 4 * rows * n))) ||
+        (src64 && (overlaps(work, wb, src64, 8 * n) || overlaps(work, wb, dst64, 8 * n)))) {
+        set_error("slt_unpermute_rays: work overlaps the data");
+        return SLT_ERR_INVALID;
+    }
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    hipStream_t st = (hipStream_t)stream;
+    RayRec* rec = static_cast<RayRec*>(work);
+    hipLaunchKernelGGL(scatter_rec_kernel, grid, block, 0, st, src, (int)rows, src64, n, order,
+                       rec);
+    hipLaunchKernelGGL(rec_to_rows_kernel, grid, block, 0, st, rec, (int)rows, n, dst, dst64);
+    return check_launch("slt_unpermute_rays");
 }
 
 }  // extern "C"

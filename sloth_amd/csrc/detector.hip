@@ -15,19 +15,12 @@
 
 namespace slt {
 
-// Row i of the state (and tau[i]) goes to output column o = order[i] when ORDERED
-// (the state is in processing order, the outputs in the caller's), else o = i.
-template <bool ORDERED>
 __global__ void __launch_bounds__(256) jones_kernel(const float* __restrict__ st, int64_t ld,
-                                                    int64_t n, int axis,
-                                                    const int32_t* __restrict__ order,
-                                                    float* __restrict__ rf,
+                                                    int64_t n, int axis, float* __restrict__ rf,
                                                     const double* __restrict__ tau,
                                                     double* __restrict__ amp) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t o = ORDERED ? (int64_t)order[i] : i;
-    if (ORDERED && (o < 0 || o >= n)) return;  // not a permutation entry
     const float x = st[i], y = st[ld + i], z = st[2 * ld + i];
     const float vx = st[3 * ld + i], vy = st[4 * ld + i], vz = st[5 * ld + i];
     float p1, a1, p2, a2;
@@ -38,11 +31,11 @@ __global__ void __launch_bounds__(256) jones_kernel(const float* __restrict__ st
     } else {
         p1 = x; a1 = atanf(vx / vz); p2 = y; a2 = atanf(vy / vz);
     }
-    rf[o] = p1;
-    rf[n + o] = a1;
-    rf[2 * n + o] = p2;
-    rf[3 * n + o] = a2;
-    if (amp && tau) amp[o] = exp(-tau[i]);
+    rf[i] = p1;
+    rf[n + i] = a1;
+    rf[2 * n + i] = p2;
+    rf[3 * n + i] = a2;
+    if (amp && tau) amp[i] = exp(-tau[i]);
 }
 
 // numpy bin index in [0, nb+1]: 0 = below, nb+1 = above (dropped), else bin+1.
@@ -64,12 +57,16 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// One histogram entry per visited ray.  ORDERED visits ray order[i] at step i
-// and lets each wavefront combine the entries it makes for one cell into a
-// single atomic per cell (count = popcount of the lanes, weight = their wave
-// sum); with rays in Morton order a wavefront's 64 rays fall in one or two
-// cells, so the device-scope atomics drop ~30-fold.  Unordered input keeps
-// one atomic per ray: its lanes rarely share a cell.
+// Cells per wavefront step whose entries are combined into one atomic each.
+constexpr int kHistCombine = 4;
+
+// One histogram entry per visited ray (ray order[i] at step i when ORDERED, else
+// ray i).  Each wavefront combines the entries it makes for its first
+// kHistCombine distinct cells into a single atomic per cell (count = popcount of
+// the lanes, weight = their wave sum): with rays in Morton order a wavefront's 64
+// rays fall in one or two cells, so the device-scope atomics on the same address
+// drop ~30-fold.  The lanes of any further cells (rays in no useful order
+// rarely share a cell) add their own entries.
 template <typename T, bool ORDERED>
 __global__ void __launch_bounds__(256) hist_kernel(const T* __restrict__ x, int64_t xs,
                                                    const T* __restrict__ y, int64_t ys,
@@ -100,17 +97,10 @@ __global__ void __launch_bounds__(256) hist_kernel(const T* __restrict__ x, int6
             if (bx >= 1 && bx <= nxb && by >= 1 && by <= nyb)
                 cell = (int64_t)(by - 1) * nxb + (bx - 1);  // H.T layout
         }
-        if (!ORDERED) {
-            if (cell >= 0) {
-                if (counts) atomicAdd(counts + cell, 1u);
-                if (wsum) atomicAdd(wsum + cell, w[j]);
-            }
-            continue;
-        }
         const double wj = (wsum && cell >= 0) ? w[j] : 0.0;
         const int c32 = (int)cell;  // cells < nxb * nyb < 2^31
         uint64_t pending = __ballot(cell >= 0);
-        while (pending) {
+        for (int k = 0; pending && k < kHistCombine; ++k) {
             const int leader = __ffsll((unsigned long long)pending) - 1;
             const int c = __builtin_amdgcn_readlane(c32, leader);
             const uint64_t same = __ballot(c32 == c);
@@ -120,6 +110,10 @@ __global__ void __launch_bounds__(256) hist_kernel(const T* __restrict__ x, int6
                 if (lane == leader) atomicAdd(wsum + c, s);
             }
             pending &= ~same;
+        }
+        if ((pending >> lane) & 1ull) {
+            if (counts) atomicAdd(counts + c32, 1u);
+            if (wsum) atomicAdd(wsum + c32, wj);
         }
     }
 }
@@ -137,25 +131,9 @@ extern "C" int slt_jones(const float* state, int64_t ld, int64_t n, int32_t prob
     }
     if (n == 0) return SLT_OK;
     const int64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(jones_kernel<false>, dim3((unsigned)blocks), dim3(256), 0,
-                       (hipStream_t)stream, state, ld, n, probing_axis, nullptr, rf, tau, amp);
+    hipLaunchKernelGGL(jones_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       state, ld, n, probing_axis, rf, tau, amp);
     return check_launch("slt_jones");
-}
-
-extern "C" int slt_jones_ordered(const float* state, int64_t ld, int64_t n,
-                                 int32_t probing_axis, const int32_t* order, float* rf,
-                                 const double* tau, double* amp, slt_stream_t stream) {
-    clear_error();
-    if (n < 0 || probing_axis < 0 || probing_axis > 2 ||
-        (n > 0 && (!state || !rf || !order)) || ld < n) {
-        set_error("slt_jones_ordered: invalid arguments");
-        return SLT_ERR_INVALID;
-    }
-    if (n == 0) return SLT_OK;
-    const int64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(jones_kernel<true>, dim3((unsigned)blocks), dim3(256), 0,
-                       (hipStream_t)stream, state, ld, n, probing_axis, order, rf, tau, amp);
-    return check_launch("slt_jones_ordered");
 }
 
 extern "C" int slt_histogram2d(const void* x, int64_t x_stride, const void* y,

@@ -7,7 +7,7 @@ from __future__ import annotations
 import torch
 
 from .. import _native as N
-from ..core.engine import axis_index, device_f32
+from ..core.engine import axis_index, device_f32, unpermute_rays
 
 
 def jones_device(state: torch.Tensor, probing_direction: str = "z", tau: torch.Tensor | None = None,
@@ -26,16 +26,10 @@ def jones_device(state: torch.Tensor, probing_direction: str = "z", tau: torch.T
     if tau is not None:
         tau = tau.to(torch.float64).contiguous()
         amp = torch.empty(n, dtype=torch.float64, device=st.device)
-    axis = axis_index(probing_direction)
-    if order is None:
-        N.check(N.lib().slt_jones(st.data_ptr(), n, n, axis, rf.data_ptr(), N.ptr(tau),
-                                  N.ptr(amp), N.stream_handle()))
-    else:
-        if order.dtype != torch.int32 or order.shape != (n,) or not order.is_contiguous():
-            raise AssertionError("order must be a contiguous int32 permutation of the rays")
-        N.check(N.lib().slt_jones_ordered(st.data_ptr(), n, n, axis, order.data_ptr(),
-                                          rf.data_ptr(), N.ptr(tau), N.ptr(amp),
-                                          N.stream_handle()))
+    N.check(N.lib().slt_jones(st.data_ptr(), n, n, axis_index(probing_direction), rf.data_ptr(),
+                              N.ptr(tau), N.ptr(amp), N.stream_handle()))
+    if order is not None:
+        rf, amp = unpermute_rays(rf, amp, order)
     return rf, amp
 
 

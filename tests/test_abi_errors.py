@@ -148,28 +148,32 @@ def test_jones_histogram_morton_rejections(L):
     _expect(L, L.slt_morton_keys(None, 0, ctypes.byref(g), None, None), N.SLT_OK, None)
 
 
-def test_gather_and_ordered_jones_rejections(L):
-    other = FAKE + 4096
-    _expect(L, L.slt_gather_rays(FAKE, -1, FAKE, other, None), N.SLT_ERR_INVALID,
+def test_ray_permutation_rejections(L):
+    other, work = FAKE + (1 << 20), FAKE + (1 << 40)
+    assert L.slt_ray_work_bytes(10) == 320 and L.slt_ray_work_bytes(0) == 0
+    assert L.slt_ray_work_bytes(-3) == 0
+    _expect(L, L.slt_gather_rays(FAKE, -1, FAKE, other, work, None), N.SLT_ERR_INVALID,
             "slt_gather_rays")
-    for s0, order, out in ((None, FAKE, other), (FAKE, None, other), (FAKE, other, None)):
-        _expect(L, L.slt_gather_rays(s0, 10, order, out, None), N.SLT_ERR_INVALID,
+    for s0, order, out, w in ((None, FAKE, other, work), (FAKE, None, other, work),
+                              (FAKE, other, None, work), (FAKE, other, FAKE, None)):
+        _expect(L, L.slt_gather_rays(s0, 10, order, out, w, None), N.SLT_ERR_INVALID,
                 "slt_gather_rays")
-    # gathering in place would overwrite rays before they are read
-    _expect(L, L.slt_gather_rays(FAKE, 10, other, FAKE, None), N.SLT_ERR_INVALID,
-            "slt_gather_rays")
-    _expect(L, L.slt_gather_rays(None, 0, None, None, None), N.SLT_OK, None)
+    # the records would overwrite rays before they are read
+    _expect(L, L.slt_gather_rays(FAKE, 10, other, FAKE + 1000, FAKE + 100, None),
+            N.SLT_ERR_INVALID, "overlaps")
+    _expect(L, L.slt_gather_rays(None, 0, None, None, None, None), N.SLT_OK, None)
 
-    _expect(L, L.slt_jones_ordered(FAKE, 10, 10, 2, None, FAKE, None, None, None),
-            N.SLT_ERR_INVALID, "slt_jones_ordered")      # order is required
-    _expect(L, L.slt_jones_ordered(FAKE, 10, 10, 3, FAKE, FAKE, None, None, None),
-            N.SLT_ERR_INVALID, "slt_jones_ordered")
-    _expect(L, L.slt_jones_ordered(FAKE, 5, 10, 2, FAKE, FAKE, None, None, None),
-            N.SLT_ERR_INVALID, "slt_jones_ordered")
-    _expect(L, L.slt_jones_ordered(FAKE, 10, -1, 2, FAKE, FAKE, None, None, None),
-            N.SLT_ERR_INVALID, "slt_jones_ordered")
-    _expect(L, L.slt_jones_ordered(None, 0, 0, 2, None, None, None, None, None), N.SLT_OK,
-            None)
+    def unperm(src=FAKE, rows=4, src64=None, n=10, order=other, dst=FAKE + 4096, dst64=None,
+               w=work):
+        return L.slt_unpermute_rays(src, rows, src64, n, order, dst, dst64, w, None)
+
+    for bad in (dict(rows=9), dict(rows=7, src64=FAKE + 8192, dst64=FAKE + 9000),
+                dict(rows=0), dict(rows=-1), dict(src64=FAKE + 8192),        # no dst64
+                dict(dst64=FAKE + 8192), dict(order=None), dict(w=None), dict(src=None),
+                dict(dst=None), dict(n=-1)):
+        _expect(L, unperm(**bad), N.SLT_ERR_INVALID, "slt_unpermute_rays")
+    _expect(L, unperm(w=FAKE + 8), N.SLT_ERR_INVALID, "overlaps")
+    _expect(L, unperm(n=0), N.SLT_OK, None)
 
 
 def test_rtm_and_beam_rejections(L):

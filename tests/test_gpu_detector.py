@@ -47,6 +47,35 @@ def test_histogram_bit_exact_vs_numpy(dtype):
     np.testing.assert_allclose(wsum.cpu().numpy(), Hw.T, rtol=1e-12, atol=1e-12)
 
 
+def test_histogram_spatially_sorted_input_combines_and_stays_exact():
+    """Rays in a spatially coherent order (sorted by cell, as after a Morton-ordered
+    trace) and no order array: the wavefront combine path, against numpy, also with a
+    mask and with more distinct cells per wavefront than are combined."""
+    import torch
+
+    from sloth_amd.processing.diagnostics import histogram2d_device
+
+    rng = np.random.default_rng(31)
+    n = 300_000
+    xe, ye = np.linspace(-1.0, 1.0, 65), np.linspace(-1.0, 1.0, 49)
+    x = rng.uniform(-1.05, 1.05, n)
+    y = rng.uniform(-1.05, 1.05, n)
+    w = rng.uniform(0, 1, n)
+    for label, key in (("sorted", np.lexsort((x, np.floor((y + 1) * 24)))),
+                       ("runs of 8", np.argsort(np.arange(n) % (n // 8), kind="stable"))):
+        xs, ys, ws = x[key], y[key], w[key]
+        mask = (np.arange(n) % 7) != 3
+        counts, wsum = histogram2d_device(torch.from_numpy(xs).cuda(),
+                                          torch.from_numpy(ys).cuda(), xe, ye,
+                                          weights=torch.from_numpy(ws),
+                                          mask=torch.from_numpy(mask))
+        H, _, _ = np.histogram2d(xs[mask], ys[mask], bins=[xe, ye])
+        Hw, _, _ = np.histogram2d(xs[mask], ys[mask], bins=[xe, ye], weights=ws[mask])
+        np.testing.assert_array_equal(counts.cpu().numpy(), H.T.astype(np.int64), err_msg=label)
+        np.testing.assert_allclose(wsum.cpu().numpy(), Hw.T, rtol=1e-12, atol=1e-12,
+                                   err_msg=label)
+
+
 def test_diagnostic_histogram_matches_numpy_and_weights_contract():
     from sloth_amd.processing.diagnostics import Diagnostic
 

@@ -289,6 +289,10 @@ def test_gathered_trace_equals_permuted_io():
     keep = torch.ones(o.numel(), dtype=torch.bool, device=o.device)
     keep[[3, 7]] = False
     assert torch.equal(g[:, keep], rays[:, o[keep]])
+    # in place: the records are taken before any row is overwritten
+    inplace = rays.clone()
+    engine.gather_rays(inplace, b.order, out=inplace)
+    assert torch.equal(inplace, rays[:, o])
     # solve: the gathered path (default) against the raw caller-order states
     dom = ScalarDomain(lengths=np.array([L, L, L]), dims=np.array(ne.shape), ne=ne,
                        probing_direction="z", inv_brems=True, Te=20.0, Z=1.0)
@@ -298,6 +302,41 @@ def test_gathered_trace_equals_permuted_io():
     rf_raw, amp_raw = jones_device(raw.states[-1], "z", raw.tau[-1])
     np.testing.assert_array_equal(rf, rf_raw.cpu().numpy())
     np.testing.assert_array_equal(amp, amp_raw.cpu().numpy())
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 100_003])
+def test_unpermute_rays_inverts_a_permutation(n):
+    """slt_unpermute_rays puts processing-order columns back: dst[:, order[t]] = src[:, t]
+    for 1-8 fp32 rows, with and without an fp64 row, also in place."""
+    import torch
+
+    from sloth_amd.core import engine
+
+    dev = torch.device("cuda", 0)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(n)
+    order = torch.randperm(n, generator=gen, device=dev).to(torch.int32)
+    o = order.long()
+    work = engine.ray_work(n, dev)
+    for rows, with64 in ((8, False), (6, True), (6, False), (4, True), (1, False), (0, True)):
+        src = (torch.randn((rows, n), generator=gen, device=dev) if rows else None)
+        src64 = torch.randn(n, generator=gen, dtype=torch.float64, device=dev) if with64 else None
+        dst, dst64 = engine.unpermute_rays(src, src64, order, work=work)
+        if rows:
+            ref = torch.empty_like(src)
+            ref[:, o] = src
+            assert torch.equal(dst, ref), (rows, with64)
+        if with64:
+            ref64 = torch.empty_like(src64)
+            ref64[o] = src64
+            assert torch.equal(dst64, ref64), (rows, with64)
+    # gather then unpermute is the identity (NaN payloads and signed zeros kept bit for bit)
+    s0 = torch.randn((6, n), generator=gen, device=dev)
+    s0[0, 0] = float("nan")
+    s0[1, n // 2] = -0.0
+    back, _ = engine.unpermute_rays(engine.gather_rays(s0, order, work=work), None, order,
+                                    work=work)
+    assert torch.equal(back.view(torch.int32), s0.view(torch.int32))
 
 
 def test_vacuum_is_exact():
