@@ -291,7 +291,8 @@ typedef struct slt_job {
     const void* packed;        /* packed grid (slt_grid_pack)                        */
     slt_grid_desc grid;
     void* state;               /* slt_ray_state_bytes(n_rays) bytes (slt_ray_state_init) */
-    const int32_t* order;      /* ray index of state row t (the init order), or NULL */
+    const int32_t* order;      /* caller's index of state row t, or NULL (outputs
+                                  then in state-row order)                         */
     int64_t n_rays;
     const int32_t* seg_nsub;   /* save schedule, as in slt_trace_args                */
     const float* seg_h;
@@ -300,7 +301,8 @@ typedef struct slt_job {
     int64_t sched_steps;       /* sum of seg_nsub                                    */
     int64_t step_begin;        /* advance every ray from this global step ...        */
     int64_t step_end;          /* ... to this one; == sched_steps finishes the job   */
-    float* out_state;          /* 6 x n_rays final state in caller order (finishing) */
+    float* out_state;          /* 6 x n_rays final state, column order[t] for row t
+                                  (row t without an order) (finishing)            */
     double* out_tau;           /* n_rays final tau, or NULL                          */
     double* out_path;          /* n_rays final path length, or NULL                  */
     uint32_t* out_steps;       /* n_rays in-box steps, or NULL                       */

@@ -216,11 +216,15 @@ class ServingEngine:
             n = mat.rays.shape[1]
             nsub, h = engine.schedule(mat.save_times, engine.nominal_step(mat.grid, self.step_fraction))
             order = engine.ray_order(mat.grid, mat.rays)
+            # the state rows in Morton order, from rays gathered into it (coalesced init);
+            # the step batches then write their outputs in the same order
+            rays_sorted = engine.gather_rays(mat.rays, order)
             state = torch.empty(int(N.lib().slt_ray_state_bytes(n)), dtype=torch.uint8,
                                 device=self.dev)
-            N.check(N.lib().slt_ray_state_init(state.data_ptr(), n, mat.rays.data_ptr(),
-                                               order.data_ptr(), ctypes.byref(mat.grid.desc),
+            N.check(N.lib().slt_ray_state_init(state.data_ptr(), n, rays_sorted.data_ptr(),
+                                               None, ctypes.byref(mat.grid.desc),
                                                N.stream_handle()))
+            del rays_sorted
             seg_nsub = engine.const_device(nsub, self.dev)
             seg_h = engine.const_device(h.astype(np.float32), self.dev)
             out_state = torch.empty((6, n), dtype=torch.float32, device=self.dev)
@@ -248,7 +252,7 @@ class ServingEngine:
         j.packed = a.mat.grid.voxels.data_ptr()
         j.grid = a.mat.grid.desc
         j.state = a.state.data_ptr()
-        j.order = a.order.data_ptr()
+        j.order = None  # outputs in state-row (Morton) order; finish_outputs puts them back
         j.n_rays = a.mat.rays.shape[1]
         j.seg_nsub = a.seg_nsub.data_ptr()
         j.seg_h = a.seg_h.data_ptr()
@@ -265,17 +269,9 @@ class ServingEngine:
 
     def _finalize(self, a: _Active):
         inv = a.mat.grid.inv_brems
-        rf, amp = jones_device(a.out_state, a.mat.grid.probing_direction,
-                               a.out_tau if inv else None)
-        counts, wsum = histogram2d_device(rf[0], rf[2], a.mat.edges, a.mat.edges, weights=amp,
-                                          order=a.order)
-        a.result = {"rf": rf if self.keep_outputs else None,
-                    "amp": amp if self.keep_outputs else None,
-                    "image": wsum if wsum is not None else counts,
-                    "path": a.out_path if self.keep_outputs else None,
-                    "state": a.out_state if self.keep_outputs else None,
-                    "tau": a.out_tau if self.keep_outputs and inv else None,
-                    "steps": a.out_steps if self.keep_outputs else None}
+        a.result = finish_outputs(a.out_state, a.out_tau if inv else None, a.out_path,
+                                  a.out_steps, a.order, a.mat.grid.probing_direction,
+                                  a.mat.edges, keep=self.keep_outputs)
         a.done = torch.cuda.Event(enable_timing=True)
         a.done.record(self.main)
 
@@ -409,6 +405,33 @@ class ServingEngine:
         return responses, stats
 
 
+def finish_outputs(state, tau, path, steps, order, probing_direction, edges, *, keep=True):
+    """A request's results from its final ray states.
+
+    state (6, N), tau (N,) or None, path (N,), steps (N,) int32 are in processing order:
+    column t belongs to the caller's ray order[t] (order None: the caller's order).  The
+    Jones vectors and the detector image are made in that order, where a wavefront's rays
+    share detector cells; with ``keep`` the state, tau, path and steps are put back in the
+    caller's order (two 32-byte records per ray) and rf / amp come from the caller-order
+    state, which gives the same values as in processing order.
+    """
+    rf, amp = jones_device(state, probing_direction, tau)
+    counts, wsum = histogram2d_device(rf[0], rf[2], edges, edges, weights=amp)
+    out = {"image": wsum if wsum is not None else counts, "rf": None, "amp": None,
+           "path": None, "state": None, "tau": None, "steps": None}
+    if not keep:
+        return out
+    if order is not None:
+        work = engine.ray_work(order.numel(), state.device)
+        state, tau = engine.unpermute_rays(state, tau, order, work=work)
+        st, path = engine.unpermute_rays(steps.view(torch.float32).unsqueeze(0), path, order,
+                                         work=work)
+        steps = st[0].view(torch.int32)
+        rf, amp = jones_device(state, probing_direction, tau)
+    out.update(rf=rf, amp=amp, path=path, state=state, tau=tau, steps=steps)
+    return out
+
+
 def run_offline(requests, *, step_fraction: float = 0.5, device=None):
     """The same requests one after another through the offline path (slt_trace).
 
@@ -422,14 +445,13 @@ def run_offline(requests, *, step_fraction: float = 0.5, device=None):
     for req in requests:
         mat = materialize(req, dev)
         res = engine.trace(mat.grid, mat.rays, mat.save_times, step_fraction=step_fraction,
-                           save_all=False, want_path=True)
-        rf, amp = jones_device(res.states[0], mat.grid.probing_direction,
-                               res.tau[0] if res.tau is not None else None)
-        counts, wsum = histogram2d_device(rf[0], rf[2], mat.edges, mat.edges, weights=amp,
-                                          order=res.order)
-        out[req.rid] = {"rf": rf, "amp": amp, "image": wsum if wsum is not None else counts,
-                        "path": res.path, "steps": res.total_steps, "state": res.states[0],
-                        "nsub": res.nsub, "h": res.h}
+                           save_all=False, want_path=True, gather=True)
+        fin = finish_outputs(res.states[0], res.tau[0] if res.tau is not None else None,
+                             res.path, res.steps, res.order if res.permuted else None,
+                             mat.grid.probing_direction, mat.edges)
+        out[req.rid] = {"rf": fin["rf"], "amp": fin["amp"], "image": fin["image"],
+                        "path": fin["path"], "steps": res.total_steps, "state": fin["state"],
+                        "steps_per_ray": fin["steps"], "nsub": res.nsub, "h": res.h}
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - t0
     for v in out.values():

@@ -59,9 +59,12 @@ class RequestSpec:
         return math.sqrt(8.0) * self.field.length / C_LIGHT
 
     def footprint_bytes(self) -> int:
-        """HBM needed while the request is resident (grid, transients, rays, outputs)."""
+        """HBM needed while the request is resident: the packed grid and the field's
+        transients (16 + 16 B per voxel); per ray the rays, order, resumable state and
+        outputs (~160 B) and, at set-up and finish, the permutation workspace and the
+        caller-order copies (~130 B); the detector image."""
         n3 = self.field.side ** 3
-        return int(16 * n3 + 16 * n3 + 200 * self.beam.n_rays + 8 * self.bins ** 2 + (1 << 20))
+        return int(16 * n3 + 16 * n3 + 288 * self.beam.n_rays + 8 * self.bins ** 2 + (1 << 20))
 
 
 def _coords(spec: FieldSpec):
