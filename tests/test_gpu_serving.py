@@ -62,6 +62,23 @@ def test_serving_mix_equals_offline_and_alone():
         _close(alone[r.rid].rf, resp[r.rid].rf)
 
 
+def test_image_only_finish_equals_full_outputs():
+    """keep_outputs=False skips the return to the caller's order: the detector image is the
+    one made with every output kept, and no per-ray outputs come back."""
+    from sloth_amd.serving import ServingEngine
+
+    full, _ = ServingEngine(max_slots=4, blocks=128, batch_work=1e6).run(_mix(4, seed=5),
+                                                                         realtime=False)
+    lean, stats = ServingEngine(max_slots=4, blocks=128, batch_work=1e6,
+                                keep_outputs=False).run(_mix(4, seed=5), realtime=False)
+    assert stats["ok"] == 4
+    for rid, r in lean.items():
+        assert r.rf is None and r.state is None and r.path is None and r.steps is None
+        assert r.ray_steps == full[rid].ray_steps
+        np.testing.assert_allclose(r.image.double().cpu().numpy(),
+                                   full[rid].image.double().cpu().numpy(), rtol=1e-12, atol=0)
+
+
 def test_graph_replay_equals_direct_launch():
     from sloth_amd.serving import ServingEngine
 
