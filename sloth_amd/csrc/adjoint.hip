@@ -186,11 +186,13 @@ __device__ __forceinline__ void scatter_corners(const DevGrid& G, float* __restr
 // wavefront's flush area in LDS: one row of 32 moments per lane (8 float4: m0,
 // mx, my, mz, mxy, mxz, myz, mxyz; rows 16-byte aligned), then kFlushSlots
 // cell sums of 32 floats (padded so that the slots start in different banks),
-// then the slots' cell ids.
+// then the slots' cell ids, their member counts and their member lists (up to
+// 64 lane ids of one byte each per slot).
 constexpr int kFlushSlots = 16;
 constexpr int kRowStride = 36;
 constexpr int kSlotStride = 33;
-constexpr int kWaveLds = 64 * kRowStride + kFlushSlots * kSlotStride + kFlushSlots;
+constexpr int kWaveLds = 64 * kRowStride + kFlushSlots * kSlotStride + 2 * kFlushSlots +
+                         kFlushSlots * 16;
 static_assert(kRowStride % 4 == 0 && kWaveLds % 4 == 0, "rows must stay 16-byte aligned");
 #define SLT_LDS __attribute__((address_space(3)))
 
@@ -200,6 +202,10 @@ struct AdjOut {
     __device__ __forceinline__ SLT_LDS float* sums() const { return lds + 64 * kRowStride; }
     __device__ __forceinline__ SLT_LDS int* ids() const {
         return reinterpret_cast<SLT_LDS int*>(lds + 64 * kRowStride + kFlushSlots * kSlotStride);
+    }
+    __device__ __forceinline__ SLT_LDS int* counts() const { return ids() + kFlushSlots; }
+    __device__ __forceinline__ SLT_LDS uint32_t* lists() const {
+        return reinterpret_cast<SLT_LDS uint32_t*>(counts() + kFlushSlots);
     }
 };
 
@@ -229,9 +235,10 @@ __device__ __forceinline__ float corner_adjoint(int E, const float m[8]) {
 // Collective flush (whole wavefront, uniform control flow): the lanes with
 // `need` set give their moments to their cell.  A sorted wavefront holds a few
 // cells at a time, and at a flush nearly all its lanes leave theirs together.
-// The flushing lanes store their moments in their LDS rows; then, cell by cell,
-// the wavefront sums the rows of the cell's lanes (component k in lanes k and
-// k + 32, which take alternate members) into a slot; then the whole wavefront
+// The flushing lanes store their moments in their LDS rows; a scalar loop over
+// the cells gives each cell a slot and lists its member lanes there; then every
+// lane sums the member rows of one (slot, component) entry at a time, four
+// member reads in flight (a fixed order per wavefront); then the whole wavefront
 // scatters the slots' corner adjoints: entry e = (slot, corner, channel) goes to
 // lane e mod 64, so a flush of c cells issues ceil(32 c / 64) full-width atomic
 // instructions.  No LDS atomics: float atomics on LDS addresses shared by many
@@ -246,34 +253,49 @@ __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, 
         for (int q = 0; q < 8; ++q) row[q] = slt_f4{src[q]->x, src[q]->y, src[q]->z, src[q]->w};
     }
     wave_lds_sync();
-    const bool upper = lane >= 32;
-    const int k = lane & 31;
+    // per cell (scalar loop): its member lanes' ids in a list, their count, the cell id
     uint64_t pending = __ballot(need);
     int nslots = 0;
     while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const int key = __builtin_amdgcn_readlane(M.id, leader);
         const bool mine = need && M.id == key;
-        uint64_t mem = __ballot(mine);
+        const uint64_t mem = __ballot(mine);
         pending &= ~mem;
         if (nslots < kFlushSlots) {
-            float acc = 0.f;
-            while (mem) {
-                const int m1 = __ffsll((unsigned long long)mem) - 1;
-                mem &= mem - 1;
-                const int m2 = mem ? __ffsll((unsigned long long)mem) - 1 : m1;
-                const bool two = mem != 0;
-                mem &= mem - 1;
-                const float v = out.lds[(upper ? m2 : m1) * kRowStride + k];
-                acc += (upper && !two) ? 0.f : v;
+            if (mine) {
+                const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(mem >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mem, 0));
+                reinterpret_cast<SLT_LDS uint8_t*>(out.lists() + nslots * 16)[pos] = (uint8_t)lane;
             }
-            acc += __shfl_xor(acc, 32);
-            if (!upper) out.sums()[nslots * kSlotStride + k] = acc;
-            if (lane == 0) out.ids()[nslots] = key;
+            if (lane == 0) {
+                out.ids()[nslots] = key;
+                out.counts()[nslots] = __popcll(mem);
+            }
             ++nslots;
         } else if (mine) {
             scatter_corners(G, out.adj, M.id, M);
         }
+    }
+    wave_lds_sync();
+    // per (slot, component) entry (lane-parallel): the sum over the slot's member rows,
+    // four member ids per LDS word and their four reads issued together
+    for (int e = lane; e < nslots * 32; e += 64) {
+        const int sl = e >> 5, k = e & 31;
+        const int cnt = out.counts()[sl];
+        const SLT_LDS uint32_t* lst = out.lists() + sl * 16;
+        float acc = 0.f;
+        for (int i = 0; i < cnt; i += 4) {
+            const uint32_t w = lst[i >> 2];
+            float v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                v[q] = out.lds[((w >> (8 * q)) & 63u) * kRowStride + k];
+                v[q] = i + q < cnt ? v[q] : 0.f;
+            }
+            acc += (v[0] + v[1]) + (v[2] + v[3]);
+        }
+        out.sums()[sl * kSlotStride + k] = acc;
     }
     wave_lds_sync();
     const int nent = nslots * 32;
