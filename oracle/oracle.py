@@ -73,8 +73,9 @@ _native = None
 def load_native() -> str:
     """Build (once per process) and load the -O3 -march=native oracle; returns its flags.
 
-    GNU C mode lets gcc contract multiply-adds into FMAs, so this build is faster and
-    not bit-identical to the checker build; it is timed, never used as the checker."""
+    GNU C mode lets gcc contract multiply-adds into FMAs, so this compile is faster and
+    not bit-identical to the one the tests check against; it is timed, never used as the
+    checker."""
     global _native
     if _native is None:
         import tempfile
