@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SLT_ABI_VERSION 2
+#define SLT_ABI_VERSION 3
 
 #define SLT_OK 0
 #define SLT_ERR_INVALID (-1)     /* bad argument (shape, pointer, range) */
@@ -49,6 +49,12 @@ typedef struct slt_grid_desc {
     float hi[3];       /* last coordinate value x[n-1], ...      (fp32)  */
     float inv_d[3];    /* (n-1) / (hi - lo)                              */
     int32_t probing_axis; /* 0, 1, 2 for x, y, z                         */
+    int32_t uniform;   /* 1: every axis a linspace up to fp32 rounding   */
+    int32_t reserved;
+    const float* knots; /* uniform = 0: device array of the n[0] + n[1] + n[2]
+                           grid points in grid units, (c[i] - lo) * inv_d
+                           per axis (x, then y, then z), from slt_grid_knots;
+                           NULL for a uniform grid                          */
 } slt_grid_desc;
 
 /* Library ABI version (SLT_ABI_VERSION). */
@@ -58,15 +64,28 @@ int slt_version(void);
 const char* slt_last_error(void);
 
 /*
- * Host-only: fill *out from the fp32 coordinate arrays (host memory).  The
- * kernels use uniform index arithmetic, so coordinates must be a uniformly
- * spaced ascending sequence up to fp32 rounding (any linspace is); otherwise
- * SLT_ERR_UNSUPPORTED.  Replaces the coordinate handling of
- * src/core/interpolator.py:72-87 (per-axis searchsorted).
+ * Host-only: fill *out from the fp32 coordinate arrays (host memory).  Each
+ * axis must be finite and strictly ascending (the reference interpolator's
+ * requirement), else SLT_ERR_INVALID.  A grid whose axes are all uniformly
+ * spaced up to fp32 rounding (any linspace is) gets uniform = 1 and index
+ * arithmetic in the kernels.  Any other grid gets uniform = 0: the caller then
+ * computes its knots with slt_grid_knots, copies them to the device and sets
+ * out->knots before slt_trace, which locates cells by a per-axis binary search
+ * over them.  The serving and adjoint entry points take uniform grids only.
+ * Replaces the coordinate handling of src/core/interpolator.py:72-87 (per-axis
+ * searchsorted).
  */
 int slt_grid_desc_init(const float* x, const float* y, const float* z, int32_t nx, int32_t ny,
                        int32_t nz, int32_t probing_axis, int32_t with_kappa,
                        slt_grid_desc* out);
+
+/*
+ * Host-only: the grid points of every axis in grid units, knots[off_a + i] =
+ * (float)(((double)c_a[i] - lo_a) * (n_a - 1) / (hi_a - lo_a)) with off = 0,
+ * n[0], n[0] + n[1] for x, y, z (n[0] + n[1] + n[2] floats, host memory).
+ */
+int slt_grid_knots(const float* x, const float* y, const float* z, const slt_grid_desc* g,
+                   float* knots);
 
 /* Bytes of the packed voxel buffer described by *g. */
 int64_t slt_grid_bytes(const slt_grid_desc* g);

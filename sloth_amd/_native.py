@@ -12,13 +12,15 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libsloth_hip.so")
 
 SLT_OK = 0
+SLT_ABI_VERSION = 3
 SLT_ERR_INVALID = -1
 SLT_ERR_HIP = -2
 SLT_ERR_UNSUPPORTED = -3
 
 # Every symbol include/sloth_hip.h declares (tests check the library exports them).
 EXPORTED = (
-    "slt_version", "slt_last_error", "slt_grid_desc_init", "slt_grid_bytes", "slt_grid_pack",
+    "slt_version", "slt_last_error", "slt_grid_desc_init", "slt_grid_knots", "slt_grid_bytes",
+    "slt_grid_pack",
     "slt_kappa", "slt_trace", "slt_jones", "slt_histogram2d", "slt_morton_keys",
     "slt_ray_state_bytes", "slt_ray_state_init", "slt_job_table_prepare", "slt_advance",
     "slt_checkpoint_bytes", "slt_tape_bytes", "slt_forward_ckpt", "slt_adjoint",
@@ -51,6 +53,9 @@ class GridDesc(ctypes.Structure):
         ("hi", ctypes.c_float * 3),
         ("inv_d", ctypes.c_float * 3),
         ("probing_axis", ctypes.c_int32),
+        ("uniform", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("knots", ctypes.c_void_p),
     ]
 
 
@@ -155,7 +160,11 @@ def lib():
     vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
     L.slt_version.restype = ctypes.c_int
     L.slt_last_error.restype = ctypes.c_char_p
+    if L.slt_version() != SLT_ABI_VERSION:
+        raise ImportError(f"{LIB_PATH} has ABI version {L.slt_version()}, these bindings "
+                          f"expect {SLT_ABI_VERSION}: rebuild the HIP library")
     L.slt_grid_desc_init.argtypes = [vp, vp, vp, i32, i32, i32, i32, i32, ctypes.POINTER(GridDesc)]
+    L.slt_grid_knots.argtypes = [vp, vp, vp, ctypes.POINTER(GridDesc), vp]
     L.slt_grid_bytes.argtypes = [ctypes.POINTER(GridDesc)]
     L.slt_grid_bytes.restype = i64
     L.slt_grid_pack.argtypes = [vp, vp, vp, vp, dbl, vp, i64, vp, i64, dbl, dbl, dbl, dbl, dbl,

@@ -146,6 +146,15 @@ class PackedGrid:
         xs, ys, zs = self.coords
         N.check(L.slt_grid_desc_init(xs.ctypes.data, ys.ctypes.data, zs.ctypes.data, nx, ny, nz,
                                      self.axis, 1 if self.inv_brems else 0, ctypes.byref(desc)))
+        if not desc.uniform:
+            # cells between the grid points: their positions in grid units, on the device
+            kn = np.empty(nx + ny + nz, dtype=np.float32)
+            N.check(L.slt_grid_knots(xs.ctypes.data, ys.ctypes.data, zs.ctypes.data,
+                                     ctypes.byref(desc), kn.ctypes.data))
+            self.knots = const_device(kn, dev, stream)
+            desc.knots = self.knots.data_ptr()
+        else:
+            self.knots = None
         self.desc = desc
         self.shape = (nx, ny, nz)
         self.nbytes = int(L.slt_grid_bytes(ctypes.byref(desc)))
@@ -166,10 +175,13 @@ class PackedGrid:
             None, None, None, None, s))
         self.device = dev
         self.spacing = np.array([1.0 / float(desc.inv_d[a]) for a in range(3)])
+        # the smallest cell width (the nominal spacing on a uniform grid)
+        self._min_cell = float(self.spacing.min()) if desc.uniform else float(
+            min(np.diff(c.astype(np.float64)).min() for c in self.coords))
 
     @property
     def min_spacing(self) -> float:
-        return float(self.spacing.min())
+        return self._min_cell
 
     def voxel_view(self) -> torch.Tensor:
         """(nx, ny, nz, 4) view in (x, y, z) index order (strided, no copy)."""

@@ -167,6 +167,7 @@ int slt_grid_desc_init(const float* x, const float* y, const float* z, int32_t n
     }
     const float* c[3] = {x, y, z};
     const int32_t n[3] = {nx, ny, nz};
+    int uniform = 1;
     for (int a = 0; a < 3; ++a) {
         if (n[a] < 2) {
             set_error("slt_grid_desc_init: axis %d has %d points (need >= 2)", a, n[a]);
@@ -177,19 +178,20 @@ int slt_grid_desc_init(const float* x, const float* y, const float* z, int32_t n
             set_error("slt_grid_desc_init: axis %d coordinates must be finite and ascending", a);
             return SLT_ERR_INVALID;
         }
+        for (int i = 1; i < n[a]; ++i) {
+            if (!((double)c[a][i] > (double)c[a][i - 1])) {
+                set_error("slt_grid_desc_init: axis %d must be strictly ascending "
+                          "(point %d is not above point %d)", a, i, i - 1);
+                return SLT_ERR_INVALID;
+            }
+        }
+        // uniform: every point within max(1e-4 cell, 4 ulp) of the linspace
         const double d = (hi - lo) / (n[a] - 1);
         const double amax = fabs(lo) > fabs(hi) ? fabs(lo) : fabs(hi);
         const double ulp = amax * 1.1920928955078125e-07;
         const double tol = fmax(1e-4 * d, 4.0 * ulp);
-        for (int i = 0; i < n[a]; ++i) {
-            const double dev = fabs((double)c[a][i] - (lo + i * d));
-            if (!(dev <= tol)) {
-                set_error("slt_grid_desc_init: axis %d is not uniformly spaced "
-                          "(point %d deviates by %.3g cells); only linspace grids are supported",
-                          a, i, dev / d);
-                return SLT_ERR_UNSUPPORTED;
-            }
-        }
+        for (int i = 0; i < n[a] && uniform; ++i)
+            if (!(fabs((double)c[a][i] - (lo + i * d)) <= tol)) uniform = 0;
         out->n[a] = n[a];
         out->lo[a] = c[a][0];
         out->hi[a] = c[a][n[a] - 1];
@@ -204,6 +206,30 @@ int slt_grid_desc_init(const float* x, const float* y, const float* z, int32_t n
     out->stride[others[0]] = (int64_t)n[probing_axis] * n[others[1]];
     out->nchan = with_kappa ? 4 : 3;
     out->probing_axis = probing_axis;
+    out->uniform = uniform;
+    out->reserved = 0;
+    out->knots = nullptr;
+    return SLT_OK;
+}
+
+int slt_grid_knots(const float* x, const float* y, const float* z, const slt_grid_desc* g,
+                   float* knots) {
+    clear_error();
+    if (!x || !y || !z || !g || !knots) {
+        set_error("slt_grid_knots: null pointer");
+        return SLT_ERR_INVALID;
+    }
+    const float* c[3] = {x, y, z};
+    int64_t off = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (g->n[a] < 2 || !(g->hi[a] > g->lo[a])) {
+            set_error("slt_grid_knots: axis %d needs >= 2 points and hi > lo", a);
+            return SLT_ERR_INVALID;
+        }
+        const double lo = g->lo[a], isp = grid_inv_spacing(g, a);
+        for (int i = 0; i < g->n[a]; ++i) knots[off + i] = (float)(((double)c[a][i] - lo) * isp);
+        off += g->n[a];
+    }
     return SLT_OK;
 }
 

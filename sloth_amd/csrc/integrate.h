@@ -30,7 +30,22 @@ struct Cell : CellPoly {
 struct FwdCell : CellPoly {
     float c0, c1, c2;
     bool face;
+    float w0, w1, w2;  // non-uniform grids: 1 / (cell width in grid units) per axis
 };
+
+// Cell of u along an axis with grid points kn[0..n) in grid units (ascending,
+// kn[0] = 0, kn[n-1] = n-1): the largest i in [0, n-2] with kn[i] <= u.
+__device__ __forceinline__ int knot_cell(const float* __restrict__ kn, int n, float u) {
+    int lo = 0, hi = n - 2;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (kn[mid] <= u)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
 
 __device__ __forceinline__ float4 f4sub(float4 a, float4 b) {
     return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
@@ -129,30 +144,47 @@ __device__ __forceinline__ unsigned unit_interval(float t) {
 
 // Interpolated (gx, gy, gz, kappa) at the grid-unit point (u0, u1, u2); zero and
 // false outside the closed box.  CHECK = false when the caller already knows the
-// point is inside.
-template <bool KAPPA, bool CHECK>
+// point is inside.  UNI: cells are unit intervals of u (uniform grid); else they
+// lie between the axis' knots and t is the fraction of the cell's width.
+template <bool KAPPA, bool CHECK, bool UNI = true>
 __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float u0, float u1, float u2,
                                          FwdCell& cc, float4& out) {
-    float tx = u0 - cc.c0;
-    float ty = u1 - cc.c1;
-    float tz = u2 - cc.c2;
+    float tx = UNI ? u0 - cc.c0 : (u0 - cc.c0) * cc.w0;
+    float ty = UNI ? u1 - cc.c1 : (u1 - cc.c1) * cc.w1;
+    float tz = UNI ? u2 - cc.c2 : (u2 - cc.c2) * cc.w2;
     bool inside = true;
     if (!(unit_interval(tx) & unit_interval(ty) & unit_interval(tz))) {
         if (CHECK && !in_box(G, u0, u1, u2)) {
             inside = false;
         } else {
-            // inside the box u >= 0, so truncation is floor
-            const int i0 = min((int)u0, G.n0 - 2);
-            const int i1 = min((int)u1, G.n1 - 2);
-            const int i2 = min((int)u2, G.n2 - 2);
-            cc.c0 = (float)i0;
-            cc.c1 = (float)i1;
-            cc.c2 = (float)i2;
+            int i0, i1, i2;
+            if (UNI) {
+                // inside the box u >= 0, so truncation is floor
+                i0 = min((int)u0, G.n0 - 2);
+                i1 = min((int)u1, G.n1 - 2);
+                i2 = min((int)u2, G.n2 - 2);
+                cc.c0 = (float)i0;
+                cc.c1 = (float)i1;
+                cc.c2 = (float)i2;
+                tx = u0 - cc.c0;
+                ty = u1 - cc.c1;
+                tz = u2 - cc.c2;
+            } else {
+                i0 = knot_cell(G.kn0, G.n0, u0);
+                i1 = knot_cell(G.kn1, G.n1, u1);
+                i2 = knot_cell(G.kn2, G.n2, u2);
+                cc.c0 = G.kn0[i0];
+                cc.c1 = G.kn1[i1];
+                cc.c2 = G.kn2[i2];
+                cc.w0 = 1.0f / (G.kn0[i0 + 1] - cc.c0);
+                cc.w1 = 1.0f / (G.kn1[i1 + 1] - cc.c1);
+                cc.w2 = 1.0f / (G.kn2[i2 + 1] - cc.c2);
+                tx = (u0 - cc.c0) * cc.w0;
+                ty = (u1 - cc.c1) * cc.w1;
+                tz = (u2 - cc.c2) * cc.w2;
+            }
             cc.face = (min(min(i0, i1), i2) == 0) | (i0 == G.n0 - 2) | (i1 == G.n1 - 2) |
                       (i2 == G.n2 - 2);
-            tx = u0 - cc.c0;
-            ty = u1 - cc.c1;
-            tz = u2 - cc.c2;
             load_cell(G, (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2)),
                       cc);
         }
@@ -289,7 +321,7 @@ struct NoTape {
 // Advances one ray from global step g_begin to g_end of the save schedule
 // (segment s has seg_nsub[s] steps of seg_h[s] seconds and ends at save s+1).
 // Saves of segments that end within [g_begin, g_end] go to the sink.
-template <bool KAPPA, bool PATH, class Tape>
+template <bool KAPPA, bool PATH, class Tape, bool UNI = true>
 __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                                           const int32_t* __restrict__ seg_nsub,
                                           const float* __restrict__ seg_h, int nseg,
@@ -298,6 +330,7 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
     FwdCell cc;
     cc.c0 = cc.c1 = cc.c2 = __builtin_nanf("");
     cc.face = true;
+    cc.w0 = cc.w1 = cc.w2 = 1.0f;
     float rbf[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) rbf[c] = (float)s.rb[c];
@@ -322,7 +355,7 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                 // The first stage doubles as the box test of the step's start point:
                 // zero RHS and false outside the box.
                 float4 A1, A2, A3, A4;
-                const bool in1 = eval_rhs<KAPPA, true>(G, rh[0], rh[1], rh[2], cc, A1);
+                const bool in1 = eval_rhs<KAPPA, true, UNI>(G, rh[0], rh[1], rh[2], cc, A1);
                 if (!in1) {
                     const int m = ballistic_steps(G, rh, vh, h, kend - k);
                     if (m > 0) {
@@ -347,13 +380,13 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                 bool in = in1;
                 const float v2x = fmaf(hh, A1.x, vh[0]), v2y = fmaf(hh, A1.y, vh[1]),
                             v2z = fmaf(hh, A1.z, vh[2]);
-                in |= eval_rhs<KAPPA, true>(G, fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]),
+                in |= eval_rhs<KAPPA, true, UNI>(G, fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]),
                                             fmaf(hh, vh[2], rh[2]), cc, A2);
                 const float v3x = fmaf(hh, A2.x, vh[0]), v3y = fmaf(hh, A2.y, vh[1]),
                             v3z = fmaf(hh, A2.z, vh[2]);
-                in |= eval_rhs<KAPPA, true>(G, fmaf(hh, v2x, rh[0]), fmaf(hh, v2y, rh[1]),
+                in |= eval_rhs<KAPPA, true, UNI>(G, fmaf(hh, v2x, rh[0]), fmaf(hh, v2y, rh[1]),
                                             fmaf(hh, v2z, rh[2]), cc, A3);
-                in |= eval_rhs<KAPPA, true>(G, fmaf(h, v3x, rh[0]), fmaf(h, v3y, rh[1]),
+                in |= eval_rhs<KAPPA, true, UNI>(G, fmaf(h, v3x, rh[0]), fmaf(h, v3y, rh[1]),
                                             fmaf(h, v3z, rh[2]), cc, A4);
                 // r += h v + h^2/6 (A1 + A2 + A3)   (= h/6 (v1 + 2 v2 + 2 v3 + v4))
                 // v += h/6 (A1 + 2 A2 + 2 A3 + A4)
@@ -393,13 +426,14 @@ __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
     }
 }
 
-template <bool KAPPA, bool PATH>
+template <bool KAPPA, bool PATH, bool UNI = true>
 __device__ __forceinline__ void integrate(const DevGrid& G, RayState& s,
                                           const int32_t* __restrict__ seg_nsub,
                                           const float* __restrict__ seg_h, int nseg,
                                           int64_t g_begin, int64_t g_end, const SaveSink& sink) {
     NoTape none;
-    integrate<KAPPA, PATH, NoTape>(G, s, seg_nsub, seg_h, nseg, g_begin, g_end, sink, none);
+    integrate<KAPPA, PATH, NoTape, UNI>(G, s, seg_nsub, seg_h, nseg, g_begin, g_end, sink,
+                                        none);
 }
 
 __device__ __forceinline__ void add_steps(unsigned long long* total, uint32_t nsteps) {

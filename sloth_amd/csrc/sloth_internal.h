@@ -37,6 +37,10 @@ struct DevGrid {
     double dsp[3];        // cell size d [m]
     double isp[3];        // 1 / d [1/m]
     int nchan;
+    // non-uniform grid: each axis' grid points in grid units (device), else null
+    const float* __restrict__ kn0;
+    const float* __restrict__ kn1;
+    const float* __restrict__ kn2;
 };
 
 // Unit conversion fields of a DevGrid from a descriptor (host or device).
@@ -65,9 +69,20 @@ __host__ __device__ inline double grid_inv_spacing(const slt_grid_desc* g, int a
     return (double)(g->n[a] - 1) / ((double)g->hi[a] - (double)g->lo[a]);
 }
 
-inline int make_dev_grid(const void* packed, const slt_grid_desc* g, DevGrid* d) {
+// Kernels built with index arithmetic take uniform grids only; slt_trace also
+// takes a non-uniform one (allow_nonuniform) whose knots are set.
+inline int make_dev_grid(const void* packed, const slt_grid_desc* g, DevGrid* d,
+                         bool allow_nonuniform = false) {
     if (!packed || !g) {
         set_error("null grid pointer");
+        return SLT_ERR_INVALID;
+    }
+    if (!g->uniform && !allow_nonuniform) {
+        set_error("this entry point takes uniformly spaced grids only (linspace axes)");
+        return SLT_ERR_UNSUPPORTED;
+    }
+    if (!g->uniform && !g->knots) {
+        set_error("non-uniform grid without knots (set slt_grid_desc.knots from slt_grid_knots)");
         return SLT_ERR_INVALID;
     }
     const int64_t total = (int64_t)g->n[0] * g->n[1] * g->n[2];
@@ -92,6 +107,9 @@ inline int make_dev_grid(const void* packed, const slt_grid_desc* g, DevGrid* d)
     fill_grid_strides(g, d);
     fill_grid_units(g, d);
     d->nchan = g->nchan;
+    d->kn0 = g->uniform ? nullptr : g->knots;
+    d->kn1 = g->uniform ? nullptr : g->knots + g->n[0];
+    d->kn2 = g->uniform ? nullptr : g->knots + g->n[0] + g->n[1];
     return SLT_OK;
 }
 
@@ -105,6 +123,7 @@ __host__ __device__ inline void make_dev_grid_device(const void* packed, const s
     fill_grid_strides(g, d);
     fill_grid_units(g, d);
     d->nchan = g->nchan;
+    d->kn0 = d->kn1 = d->kn2 = nullptr;  // the host check admits uniform grids only
 }
 
 // Device memory seen as the global address space.  A pointer that a kernel

@@ -58,7 +58,7 @@ struct TraceK {
 // Five wavefronts per SIMD (96 VGPRs): the few values that do not fit spill
 // only outside the step loop, and the extra wavefront hides more of the
 // gather latency than the spills cost (50.5 vs 52.7 ms for 1e10 ray-steps).
-template <bool KAPPA, bool PATH>
+template <bool KAPPA, bool PATH, bool UNI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 trace_kernel(TraceK K) {
     const int lb = xcd_block(blockIdx.x, gridDim.x, K.nblocks);
@@ -75,7 +75,7 @@ trace_kernel(TraceK K) {
         sink.put(0, s, K.G);
         int64_t total = 0;
         for (int q = 0; q < K.nseg; ++q) total += K.seg_nsub[q];
-        integrate<KAPPA, PATH>(K.G, s, K.seg_nsub, K.seg_h, K.nseg, 0, total, sink);
+        integrate<KAPPA, PATH, UNI>(K.G, s, K.seg_nsub, K.seg_h, K.nseg, 0, total, sink);
         if (PATH && K.path) K.path[ray] = s.path;
         nsteps = inbox_steps(s);
         if (K.steps) K.steps[ray] = nsteps;
@@ -268,7 +268,7 @@ extern "C" int slt_trace(const void* packed, const slt_grid_desc* g, const slt_t
                          slt_stream_t stream) {
     clear_error();
     TraceK K;
-    int rc = make_dev_grid(packed, g, &K.G);
+    int rc = make_dev_grid(packed, g, &K.G, /*allow_nonuniform=*/true);
     if (rc) return rc;
     if (!a || a->n_rays < 0) {
         set_error("slt_trace: invalid arguments");
@@ -305,14 +305,26 @@ extern "C" int slt_trace(const void* packed, const slt_grid_desc* g, const slt_t
     const bool kap = g->nchan >= 4;
     const bool pth = a->path != nullptr;
     hipStream_t s = (hipStream_t)stream;
-    if (kap && pth)
-        hipLaunchKernelGGL((trace_kernel<true, true>), dim3(grid), dim3(256), 0, s, K);
-    else if (kap)
-        hipLaunchKernelGGL((trace_kernel<true, false>), dim3(grid), dim3(256), 0, s, K);
-    else if (pth)
-        hipLaunchKernelGGL((trace_kernel<false, true>), dim3(grid), dim3(256), 0, s, K);
-    else
-        hipLaunchKernelGGL((trace_kernel<false, false>), dim3(grid), dim3(256), 0, s, K);
+    const dim3 gr(grid), bl(256);
+    if (g->uniform) {
+        if (kap && pth)
+            hipLaunchKernelGGL((trace_kernel<true, true, true>), gr, bl, 0, s, K);
+        else if (kap)
+            hipLaunchKernelGGL((trace_kernel<true, false, true>), gr, bl, 0, s, K);
+        else if (pth)
+            hipLaunchKernelGGL((trace_kernel<false, true, true>), gr, bl, 0, s, K);
+        else
+            hipLaunchKernelGGL((trace_kernel<false, false, true>), gr, bl, 0, s, K);
+    } else {
+        if (kap && pth)
+            hipLaunchKernelGGL((trace_kernel<true, true, false>), gr, bl, 0, s, K);
+        else if (kap)
+            hipLaunchKernelGGL((trace_kernel<true, false, false>), gr, bl, 0, s, K);
+        else if (pth)
+            hipLaunchKernelGGL((trace_kernel<false, true, false>), gr, bl, 0, s, K);
+        else
+            hipLaunchKernelGGL((trace_kernel<false, false, false>), gr, bl, 0, s, K);
+    }
     return check_launch("slt_trace");
 }
 

@@ -280,6 +280,38 @@ def test_adjoint_rejections(L):
     _expect(L, unpack(g=short), N.SLT_ERR_INVALID, "slt_adjoint_unpack")
 
 
+def test_nonuniform_grids_only_where_supported(L):
+    """A non-uniform grid needs its knots for slt_trace; the adjoint and the serving
+    step batches, built on index arithmetic, refuse it before any device work."""
+    g = _desc(L)
+    nu = _copy(g)
+    nu.uniform, nu.knots = 0, None
+    a = N.TraceArgs()
+    a.n_rays, a.s0, a.n_seg, a.seg_nsub, a.seg_h, a.save_state = 0, FAKE, 1, FAKE, FAKE, FAKE
+    _expect(L, L.slt_trace(FAKE, ctypes.byref(nu), ctypes.byref(a), None), N.SLT_ERR_INVALID,
+            "knots")
+    nu.knots = FAKE
+    _expect(L, L.slt_trace(FAKE, ctypes.byref(nu), ctypes.byref(a), None), N.SLT_OK, None)
+    aa = N.AdjointArgs()
+    aa.n_rays, aa.s0, aa.n_seg, aa.ckpt_every = 10, FAKE, 1, 16
+    aa.seg_nsub, aa.seg_h, aa.sched_steps, aa.checkpoints = FAKE, FAKE, 100, FAKE
+    aa.final_state, aa.seed_state, aa.grid_adj, aa.tape = FAKE, FAKE, FAKE, FAKE
+    for fn in (L.slt_forward_ckpt, L.slt_adjoint):
+        _expect(L, fn(FAKE, ctypes.byref(nu), ctypes.byref(aa), None), N.SLT_ERR_UNSUPPORTED,
+                "uniformly spaced")
+    aa.tape = None
+    _expect(L, L.slt_adjoint_noalloc(FAKE, ctypes.byref(nu), ctypes.byref(aa), None),
+            N.SLT_ERR_UNSUPPORTED, "uniformly spaced")
+    t = N.JobTable()
+    t.n_jobs = 1
+    j = t.jobs[0]
+    j.packed, j.grid, j.state, j.n_rays = FAKE, nu, FAKE, 10
+    j.seg_nsub, j.seg_h, j.n_seg, j.sched_steps, j.step_begin, j.step_end = FAKE, FAKE, 1, 10, 0, 10
+    j.out_state, j.out_path, j.out_steps = FAKE, FAKE, FAKE
+    _expect(L, L.slt_job_table_prepare(ctypes.byref(t)), N.SLT_ERR_UNSUPPORTED,
+            "uniformly spaced")
+
+
 def test_sizes_of_invalid_requests_are_zero(L):
     assert L.slt_ray_state_bytes(-1) == 0
     assert L.slt_checkpoint_bytes(-1, 10, 16) == 0

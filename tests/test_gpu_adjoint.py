@@ -369,3 +369,22 @@ def test_adjoint_full_config_directional_derivative():
         print(f"config A direction {seed}: <grad, delta> {ad:.6e}, central difference "
               f"{fd:.6e}, relative {rel:.2e}", flush=True)
         assert rel <= 2e-3
+
+
+def test_adjoint_refuses_nonuniform_grid():
+    """The adjoint kernels locate cells by index arithmetic: a non-uniform grid is refused
+    with NotImplementedError before any device work."""
+    import torch
+
+    from sloth_amd.adjoint import trace_states
+
+    n, L = 12, 4e-3
+    c = np.float32(np.linspace(-L / 2, L / 2, n))
+    z = c.copy()
+    z[5] += np.float32(0.2 * L / (n - 1))
+    ne = torch.full((n, n, n), 1e24, dtype=torch.float32, device="cuda")
+    s0 = torch.zeros((6, 4), dtype=torch.float32, device="cuda")
+    s0[2] = -L / 2
+    s0[5] = 3e8
+    with pytest.raises(NotImplementedError, match="uniformly spaced"):
+        trace_states(ne, s0, c, c, z, lwl=1064e-9, ckpt_every=4)

@@ -339,6 +339,72 @@ def test_unpermute_rays_inverts_a_permutation(n):
     assert torch.equal(back.view(torch.int32), s0.view(torch.int32))
 
 
+def _nonuniform_axes(n, L, seed):
+    """Three strictly ascending axes over [-L/2, L/2]: sinh-stretched (dense in the middle),
+    linspace jittered by up to 0.3 cell, and geometric (each cell 3 % wider than the last)."""
+    rng = np.random.default_rng(seed)
+    s = np.linspace(-1.0, 1.0, n)
+    x = (L / 2) * np.sinh(1.5 * s) / np.sinh(1.5)
+    y = s * (L / 2) + rng.uniform(-0.3, 0.3, n) * (L / (n - 1))
+    y[0], y[-1] = -L / 2, L / 2
+    w = 1.03 ** np.arange(n - 1)
+    z = -L / 2 + L * np.concatenate([[0.0], np.cumsum(w) / w.sum()])
+    return tuple(np.float32(a) for a in (x, y, z))
+
+
+@pytest.mark.parametrize("pd", ["z", "x"])
+def test_trace_nonuniform_grid_matches_oracle(orc, pd):
+    """Non-uniform axes (the reference interpolator's searchsorted per axis, np.gradient
+    with coordinate arrays): the kernel's knot search against the FP64 oracle."""
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import kolmogorov_field
+
+    L = 6e-3
+    ne, _ = kolmogorov_field(48, L, seed=13, n0=4e24, frac=0.3, l_min=0.5e-3)
+    coords = _nonuniform_axes(48, L, 14)
+    s0 = Beam(2000, 2e-3, 1e-3, L / 2, probing_direction=pd, beam_type="circular",
+              seeded=True).s0
+    t = np.linspace(0, 1.5 * L / C, 4)
+    gpu = gpu_trace(ne, coords, s0, t, lwl=532e-9, inv_brems=True, Te=30.0, Z=1.0,
+                    probing_direction=pd)
+    assert gpu["grid"].desc.uniform == 0
+    # the step is half the smallest cell
+    assert gpu["h"].max() * C <= 0.5 * min(np.diff(c.astype(np.float64)).min()
+                                           for c in coords) * (1 + 1e-6)
+    ref = oracle_trace(orc, ne, coords, s0, gpu["nsub"], gpu["h"], lwl=532e-9, inv_brems=True,
+                       Te=30.0, Z=1.0)
+    print("non-uniform", pd, assert_parity(gpu, ref, coords))
+
+
+def test_nonuniform_trough_closed_form(orc):
+    """The quadratic trough on non-uniform axes: numpy.gradient's non-uniform formula is
+    exact for a quadratic at interior points and trilinear interpolation is exact for the
+    linear gradient, so rays kept out of the edge cells follow x0 cos(w t) as on a uniform
+    grid; also against the oracle."""
+    from sloth_amd.core.beam import Beam
+
+    L, n, n0, R = 12e-3, 96, 5e24, 3e-3
+    coords = _nonuniform_axes(n, L, 22)
+    x, y = (c.astype(np.float64) for c in coords[:2])
+    prof = n0 * (1.0 + (x[:, None] ** 2 + y[None, :] ** 2) / R ** 2)
+    ne = np.ascontiguousarray(np.broadcast_to(prof[:, :, None], (n, n, n)), dtype=np.float32)
+    # the outermost cells (edge-order-1 gradient) lie beyond 5.7 mm on every axis
+    assert min(min(-c[1], c[-2]) for c in coords[:2]) > 4.2e-3
+    s0 = Beam(4000, 4e-3, 0.0, L / 2, beam_type="square", seeded=True).s0
+    t = np.linspace(0, 11e-3, 6) / C
+    gpu = gpu_trace(ne, coords, s0, t, lwl=532e-9)
+    assert gpu["grid"].desc.uniform == 0
+    ref = oracle_trace(orc, ne, coords, s0, gpu["nsub"], gpu["h"], lwl=532e-9)
+    print("non-uniform trough parity", assert_parity(gpu, ref, coords))
+    omega = 2 * np.pi * C / 532e-9
+    w = C * np.sqrt(n0 / (3.14207787e-4 * omega ** 2)) / R
+    x0, y0 = s0[0].astype(np.float64), s0[1].astype(np.float64)
+    fin = gpu["states"][-1]
+    # grid points sit at their fp32 positions in grid units: ~1e-9 m of slack
+    np.testing.assert_allclose(fin[0], x0 * np.cos(w * t[-1]), rtol=0, atol=3e-9)
+    np.testing.assert_allclose(fin[1], y0 * np.cos(w * t[-1]), rtol=0, atol=3e-9)
+
+
 def test_vacuum_is_exact():
     ne = np.zeros((12, 12, 12), np.float32)
     c = np.float32(np.linspace(-2e-3, 2e-3, 12))

@@ -33,7 +33,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert not missing, missing
     assert set(N.EXPORTED) == set(declared)
     L = N.lib()
-    assert L.slt_version() == 2
+    assert L.slt_version() == N.SLT_ABI_VERSION == 3
     assert L.slt_last_error() in (b"", None) or isinstance(L.slt_last_error(), bytes)
 
 
@@ -65,14 +65,32 @@ def test_grid_desc_layout_probing_axis_fastest():
     assert N.lib().slt_grid_bytes(ctypes.byref(d)) == 5 * 6 * 7 * 16
 
 
-def test_grid_desc_rejects_bad_coordinates():
+def test_grid_desc_classifies_and_rejects_coordinates():
+    """A linspace (up to fp32 rounding) is uniform; any other strictly ascending axis
+    makes the grid non-uniform, with knots in grid units; an axis that is not strictly
+    ascending is rejected, as the reference interpolator rejects it."""
     from sloth_amd import _native as N
 
     c = np.float32(np.linspace(0, 1, 8))
+    rc, d = _desc(c, c, c)
+    assert rc == 0 and d.uniform == 1 and not d.knots
+    wobble = c.copy()
+    wobble[3] += np.float32(1e-7)                    # within fp32 rounding of the linspace
+    assert _desc(wobble, c, c)[1].uniform == 1
     bad = c.copy()
     bad[3] += 0.05
-    assert _desc(bad, c, c)[0] == N.SLT_ERR_UNSUPPORTED
-    assert "uniformly" in N.last_error()
+    rc, d = _desc(bad, c, c)
+    assert rc == 0 and d.uniform == 0 and not d.knots
+    kn = np.empty(24, dtype=np.float32)
+    assert N.lib().slt_grid_knots(bad.ctypes.data, c.ctypes.data, c.ctypes.data,
+                                  ctypes.byref(d), kn.ctypes.data) == 0
+    ref = np.concatenate([((bad.astype(np.float64) - bad[0]) * 7 / (float(bad[-1]) - bad[0])),
+                          np.arange(8.0), np.arange(8.0)]).astype(np.float32)
+    np.testing.assert_array_equal(kn, ref)
+    assert kn[0] == 0 and kn[7] == 7 and kn[3] == pytest.approx(3 + 0.05 * 7, rel=1e-6)
+    flat = c.copy()
+    flat[4] = flat[3]                                # not strictly ascending
+    assert _desc(flat, c, c)[0] == N.SLT_ERR_INVALID and "strictly" in N.last_error()
     assert _desc(c[:1], c, c)[0] == N.SLT_ERR_INVALID
     assert _desc(c[::-1].copy(), c, c)[0] == N.SLT_ERR_INVALID
     assert _desc(c, c, c, axis=3)[0] == N.SLT_ERR_INVALID
