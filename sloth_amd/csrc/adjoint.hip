@@ -185,23 +185,19 @@ __device__ __forceinline__ void scatter_corners(const DevGrid& G, float* __restr
 // Where flushed moments go: the voxel adjoint in global memory and this
 // wavefront's flush area in LDS: one row of 32 moments per lane (8 float4: m0,
 // mx, my, mz, mxy, mxz, myz, mxyz; rows 16-byte aligned), then kFlushSlots
-// cell sums of 32 floats (padded so that the slots start in different banks),
-// then the slots' cell ids, their member counts and their member lists (up to
-// 64 lane ids of one byte each per slot).
+// cell ids, their member counts and their member lists (up to 64 lane ids of one
+// byte each per slot).
 constexpr int kFlushSlots = 16;
 constexpr int kRowStride = 36;
-constexpr int kSlotStride = 33;
-constexpr int kWaveLds = 64 * kRowStride + kFlushSlots * kSlotStride + 2 * kFlushSlots +
-                         kFlushSlots * 16;
+constexpr int kWaveLds = 64 * kRowStride + 2 * kFlushSlots + kFlushSlots * 16;
 static_assert(kRowStride % 4 == 0 && kWaveLds % 4 == 0, "rows must stay 16-byte aligned");
 #define SLT_LDS __attribute__((address_space(3)))
 
 struct AdjOut {
     float* __restrict__ adj;
     SLT_LDS float* lds;
-    __device__ __forceinline__ SLT_LDS float* sums() const { return lds + 64 * kRowStride; }
     __device__ __forceinline__ SLT_LDS int* ids() const {
-        return reinterpret_cast<SLT_LDS int*>(lds + 64 * kRowStride + kFlushSlots * kSlotStride);
+        return reinterpret_cast<SLT_LDS int*>(lds + 64 * kRowStride);
     }
     __device__ __forceinline__ SLT_LDS int* counts() const { return ids() + kFlushSlots; }
     __device__ __forceinline__ SLT_LDS uint32_t* lists() const {
@@ -216,31 +212,16 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The adjoint of corner E (bits: 4 = x, 2 = y, 1 = z) of a cell from the 8
-// moments of one channel (m0, mx, my, mz, mxy, mxz, myz, mxyz): corner E has
-// weight prod_a (E_a ? t_a : 1 - t_a), so its adjoint is
-// sum over monomials S containing E of (-1)^|S \ E| m_S.
-__device__ __forceinline__ float corner_adjoint(int E, const float m[8]) {
-    constexpr int kSet[8] = {0, 4, 2, 1, 6, 5, 3, 7};
-    float v = 0.f;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int S = kSet[q];
-        const float w = (S & E) == E ? ((__popc(S ^ E) & 1) ? -1.f : 1.f) : 0.f;
-        v = fmaf(w, m[q], v);
-    }
-    return v;
-}
-
 // Collective flush (whole wavefront, uniform control flow): the lanes with
 // `need` set give their moments to their cell.  A sorted wavefront holds a few
 // cells at a time, and at a flush nearly all its lanes leave theirs together.
 // The flushing lanes store their moments in their LDS rows; a scalar loop over
 // the cells gives each cell a slot and lists its member lanes there; then every
-// lane sums the member rows of one (slot, component) entry at a time, four
-// member reads in flight (a fixed order per wavefront); then the whole wavefront
-// scatters the slots' corner adjoints: entry e = (slot, corner, channel) goes to
-// lane e mod 64, so a flush of c cells issues ceil(32 c / 64) full-width atomic
+// lane sums the member rows of one (slot, channel, monomial) entry at a time, four
+// member reads in flight (a fixed order per wavefront), turns the eight monomial
+// sums of its (slot, channel) into the eight corner adjoints with three butterfly
+// steps across the lanes that hold them, and adds its corner's value to the voxel
+// adjoint: a flush of c cells issues ceil(32 c / 64) full-width atomic
 // instructions.  No LDS atomics: float atomics on LDS addresses shared by many
 // lanes serialise.  Lanes of cells beyond the slots scatter their own.
 __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, Moments& M,
@@ -278,10 +259,15 @@ __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, 
         }
     }
     wave_lds_sync();
-    // per (slot, component) entry (lane-parallel): the sum over the slot's member rows,
-    // four member ids per LDS word and their four reads issued together
+    // per (slot, channel, monomial S) entry (lane-parallel), the eight monomials of a
+    // channel in eight adjacent lanes: the sum over the slot's member rows, four member
+    // reads in flight; then the corner adjoints by the superset Moebius transform across
+    // those lanes (corner E = sum over S containing E of (-1)^|S \ E| m_S), one butterfly
+    // per axis bit; each lane adds its corner's value to the voxel adjoint.
     for (int e = lane; e < nslots * 32; e += 64) {
-        const int sl = e >> 5, k = e & 31;
+        const int sl = e >> 5, ch = (e >> 3) & 3, S = e & 7;
+        const int q = (0x74516230 >> (4 * S)) & 7;  // the moment of monomial set S
+        const int k = 4 * q + ch;
         const int cnt = out.counts()[sl];
         const SLT_LDS uint32_t* lst = out.lists() + sl * 16;
         float acc = 0.f;
@@ -289,26 +275,19 @@ __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, 
             const uint32_t w = lst[i >> 2];
             float v[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                v[q] = out.lds[((w >> (8 * q)) & 63u) * kRowStride + k];
-                v[q] = i + q < cnt ? v[q] : 0.f;
+            for (int r = 0; r < 4; ++r) {
+                v[r] = out.lds[((w >> (8 * r)) & 63u) * kRowStride + k];
+                v[r] = i + r < cnt ? v[r] : 0.f;
             }
             acc += (v[0] + v[1]) + (v[2] + v[3]);
         }
-        out.sums()[sl * kSlotStride + k] = acc;
-    }
-    wave_lds_sync();
-    const int nent = nslots * 32;
-    for (int e = lane; e < nent; e += 64) {
-        const int sl = e >> 5, corner = (e >> 2) & 7, ch = e & 3;
-        const SLT_LDS float* p = out.sums() + sl * kSlotStride + ch;
-        float m[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) m[q] = p[4 * q];
-        const float v = corner_adjoint(corner, m);
-        const int off =
-            ((corner >> 2) & 1) * G.s0 + ((corner >> 1) & 1) * G.s1 + (corner & 1) * G.s2;
-        atomicAdd(out.adj + (int64_t)(out.ids()[sl] + off) * 4 + ch, v);
+        for (int bit = 1; bit < 8; bit <<= 1) {
+            const float partner = __shfl_xor(acc, bit);
+            acc = (S & bit) ? acc : acc - partner;
+        }
+        const int off = ((S >> 2) & 1) * G.s0 + ((S >> 1) & 1) * G.s1 + (S & 1) * G.s2;
+        atomicAdd(out.adj + (int64_t)(out.ids()[sl] + off) * 4 + ch, acc);
     }
     wave_lds_sync();
 }
