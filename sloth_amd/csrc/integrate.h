@@ -34,9 +34,26 @@ struct FwdCell : CellPoly {
 };
 
 // Cell of u along an axis with grid points kn[0..n) in grid units (ascending,
-// kn[0] = 0, kn[n-1] = n-1): the largest i in [0, n-2] with kn[i] <= u.
+// kn[0] = 0, kn[n-1] = n-1): the largest i in [0, n-2] with kn[i] <= u.  In grid
+// units floor(u) is the cell of the linspace, so the search starts there and walks
+// up to kKnotWalk cells each way (enough for grids whose points stray a few cells
+// from the linspace), then falls back to a binary search of what is left.
+constexpr int kKnotWalk = 4;
 __device__ __forceinline__ int knot_cell(const float* __restrict__ kn, int n, float u) {
-    int lo = 0, hi = n - 2;
+    int i = min(max((int)u, 0), n - 2);
+    int lo = 0, hi = n - 2;  // the answer stays in [lo, hi]
+#pragma unroll 1
+    for (int k = 0; k < kKnotWalk; ++k) {
+        if (kn[i] > u) {
+            hi = i - 1;
+            i = max(i - 1, lo);
+        } else if (i < n - 2 && kn[i + 1] <= u) {
+            lo = i + 1;
+            i = lo;
+        } else {
+            return i;
+        }
+    }
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (kn[mid] <= u)

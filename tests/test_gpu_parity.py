@@ -192,13 +192,17 @@ def test_trace_other_probing_axes(orc, pd):
     print(pd, assert_parity(gpu, ref, coords))
 
 
-def test_trace_oblique_rays_outside_starts_and_side_exits(orc):
-    """Rays starting outside the box, entering through any face, missing it, leaving sideways."""
+@pytest.mark.parametrize("uniform", [True, False])
+def test_trace_oblique_rays_outside_starts_and_side_exits(orc, uniform):
+    """Rays starting outside the box, entering through any face, missing it, leaving sideways,
+    on a uniform grid and on non-uniform axes (the knot search on every entry)."""
     rng = np.random.default_rng(11)
     L = 4e-3
     from sloth_amd.fields import kolmogorov_field
 
     ne, coords = kolmogorov_field(40, L, seed=7, n0=8e24, frac=0.4, l_min=0.4e-3)
+    if not uniform:
+        coords = _nonuniform_axes(40, L, 8)
     n = 3000
     # start on a sphere of radius 4 mm aimed at random points near the box
     d = rng.normal(size=(3, n))
@@ -212,7 +216,8 @@ def test_trace_oblique_rays_outside_starts_and_side_exits(orc):
     gpu = gpu_trace(ne, coords, s0, t, lwl=532e-9, inv_brems=True, Te=30.0, Z=1.0)
     ref = oracle_trace(orc, ne, coords, s0, gpu["nsub"], gpu["h"], lwl=532e-9, inv_brems=True,
                        Te=30.0, Z=1.0)
-    print("oblique", assert_parity(gpu, ref, coords, step_mismatch_frac=5e-3))
+    assert gpu["grid"].desc.uniform == (1 if uniform else 0)
+    print("oblique", uniform, assert_parity(gpu, ref, coords, step_mismatch_frac=5e-3))
     assert (gpu["steps"] == 0).any() and (gpu["steps"] > 0).any()
 
 
