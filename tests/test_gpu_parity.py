@@ -410,6 +410,42 @@ def test_nonuniform_trough_closed_form(orc):
     np.testing.assert_allclose(fin[1], y0 * np.cos(w * t[-1]), rtol=0, atol=3e-9)
 
 
+def test_solve_with_a_nonuniform_duck_typed_domain(orc):
+    """solve() on a bare object with the ScalarDomain attributes and non-uniform axes (the
+    reference's tests pass such objects): rf and amp equal the raw trace's Jones vectors bit
+    for bit, and the final states of a ray subset match the oracle."""
+    from types import SimpleNamespace
+
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.core.propagator import solve
+    from sloth_amd.fields import kolmogorov_field
+    from sloth_amd.shared.propagation import jones_device
+
+    L = 6e-3
+    ne, _ = kolmogorov_field(40, L, seed=17, n0=3e24, frac=0.3, l_min=0.5e-3)
+    x, y, z = _nonuniform_axes(40, L, 18)
+    dom = SimpleNamespace(ne=ne, x=x, y=y, z=z, probing_direction="z",
+                          lengths=np.float32([L, L, L]), dims=np.int32(ne.shape),
+                          inv_brems=True, Te=np.float32(25.0), Z=np.float32(1.0),
+                          Np_total=None, region_count=1, ray_batch_count=1)
+    s0 = Beam(20000, 2e-3, 0.0, L / 2, beam_type="square", seeded=True).s0
+    rf, amp, _ = solve(s0, dom, L, lwl=532e-9, verbose=False)
+    raw, _, _ = solve(s0, dom, L, lwl=532e-9, verbose=False, return_raw_results=True)
+    assert raw.states.shape[0] == 1 and not raw.permuted
+    rf_raw, amp_raw = jones_device(raw.states[-1], "z", raw.tau[-1])
+    np.testing.assert_array_equal(rf, rf_raw.cpu().numpy())
+    np.testing.assert_array_equal(amp, amp_raw.cpu().numpy())
+    # a ray subset of the same trace against the oracle
+    idx = np.random.default_rng(19).choice(s0.shape[1], 2000, replace=False)
+    gpu = {"states": raw.states.cpu().numpy().astype(np.float64)[:, :, idx],
+           "tau": raw.tau.cpu().numpy()[:, idx], "path": None,
+           "steps": raw.steps.cpu().numpy().astype(np.int64)[idx]}
+    ref = oracle_trace(orc, ne, (x, y, z), s0[:, idx], raw.nsub, raw.h, lwl=532e-9,
+                       inv_brems=True, Te=25.0, Z=1.0)
+    ref["saves"] = ref["saves"][-1:]
+    print("solve non-uniform", assert_parity(gpu, ref, (x, y, z)))
+
+
 def test_vacuum_is_exact():
     ne = np.zeros((12, 12, 12), np.float32)
     c = np.float32(np.linspace(-2e-3, 2e-3, 12))
