@@ -86,10 +86,10 @@ def export_pvti(arr: np.ndarray, fname: str = None, extent_x=None, extent_y=None
     cs = [np.linspace(-ext[i], ext[i], shape[i]) for i in range(3)]
     size = [np.max(cs[i]) / (shape[i] // 2) for i in range(3)]
     _write_vti(f"{fname}.vti", rnec, size)
-    print(f"VTI saved under {fname}.vti")
+    print(f"wrote {fname}.vti (cell data rnec)")
     _write_pvti(f"{fname}.pvti", shape, size, fname.split("/")[-1] + ".vti",
                 "Float32" if rnec.dtype == np.float32 else "Float64")
-    print(f"Scalar Domain electron density succesfully saved under {fname}.pvti !")
+    print(f"wrote {fname}.pvti (parallel header for {fname}.vti)")
 
 
 def _write_pvti(path, shape, spacing, source, vtype):

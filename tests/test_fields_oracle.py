@@ -140,7 +140,7 @@ def test_export_pvti_round_trip(tmp_path, dtype, capsys):
     assert shape == (5, 4, 3) and img.dtype == dtype
     np.testing.assert_array_equal(img, a)
     np.testing.assert_allclose(spacing, [2.5 / 2, 2.0 / 2, 1.5 / 1])
-    assert "saved under" in capsys.readouterr().out
+    assert "wrote " in capsys.readouterr().out
 
 
 def test_pvti_readin_documented_encodings(tmp_path):
