@@ -151,35 +151,34 @@ __device__ __forceinline__ float4 f4add(float4 a, float4 b) {
     return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 
-__device__ __forceinline__ void f4atomic(float* p, float4 v) {
-    atomicAdd(p + 0, v.x);
-    atomicAdd(p + 1, v.y);
-    atomicAdd(p + 2, v.z);
-    atomicAdd(p + 3, v.w);
-}
+#define SLT_LDS __attribute__((address_space(3)))
 
-// Corner adjoints of summed moments: corner (ex, ey, ez) has weight
-// prod_a (e_a ? t_a : 1 - t_a), whose monomial expansion gives these sums.
-__device__ __forceinline__ void scatter_corners(const DevGrid& G, float* __restrict__ adj, int id,
-                                                const Moments& M) {
-    const float4 c111 = M.mxyz;
-    const float4 c110 = f4sub(M.mxy, M.mxyz);
-    const float4 c101 = f4sub(M.mxz, M.mxyz);
-    const float4 c011 = f4sub(M.myz, M.mxyz);
-    const float4 c100 = f4sub(f4sub(M.mx, M.mxy), c101);
-    const float4 c010 = f4sub(f4sub(M.my, M.mxy), c011);
-    const float4 c001 = f4sub(f4sub(M.mz, M.mxz), c011);
-    const float4 c000 = f4sub(f4sub(f4sub(M.m0, M.mx), f4add(M.my, M.mz)),
-                              f4sub(f4sub(f4sub(M.mxyz, M.mxy), M.mxz), M.myz));
-    float* p = adj + (int64_t)id * 4;
-    f4atomic(p, c000);
-    f4atomic(p + 4 * G.s2, c001);
-    f4atomic(p + 4 * G.s1, c010);
-    f4atomic(p + 4 * (G.s1 + G.s2), c011);
-    f4atomic(p + 4 * G.s0, c100);
-    f4atomic(p + 4 * (G.s0 + G.s2), c101);
-    f4atomic(p + 4 * (G.s0 + G.s1), c110);
-    f4atomic(p + 4 * (G.s0 + G.s1 + G.s2), c111);
+// Corner adjoints of one lane's moments, read back from its LDS row (8 float4:
+// m0, mx, my, mz, mxy, mxz, myz, mxyz) one channel at a time: corner E (axis
+// bits x = 4, y = 2, z = 1) = sum over monomial sets S containing E of
+// (-1)^|S \ E| m_S, three butterfly steps over the 8 moments; each is added to
+// its voxel's adjoint.  Used for the cells of a flush beyond its slots, so the
+// moment registers are not read here and nothing of this rare path is kept
+// live across the flush's cell loop.
+__device__ __forceinline__ void scatter_row(const DevGrid& G, float* __restrict__ adj, int id,
+                                            const SLT_LDS float* row) {
+#pragma unroll 1
+    for (int ch = 0; ch < 4; ++ch) {
+        float m[8];
+#pragma unroll
+        for (int S = 0; S < 8; ++S) m[S] = row[4 * ((0x74516230 >> (4 * S)) & 7) + ch];
+#pragma unroll
+        for (int b = 1; b < 8; b <<= 1) {
+#pragma unroll
+            for (int S = 0; S < 8; ++S)
+                if (!(S & b)) m[S] -= m[S | b];
+        }
+#pragma unroll
+        for (int S = 0; S < 8; ++S) {
+            const int off = ((S >> 2) & 1) * G.s0 + ((S >> 1) & 1) * G.s1 + (S & 1) * G.s2;
+            atomicAdd(adj + (int64_t)(id + off) * 4 + ch, m[S]);
+        }
+    }
 }
 
 // Where flushed moments go: the voxel adjoint in global memory and this
@@ -191,7 +190,6 @@ constexpr int kFlushSlots = 16;
 constexpr int kRowStride = 36;
 constexpr int kWaveLds = 64 * kRowStride + 2 * kFlushSlots + kFlushSlots * 16;
 static_assert(kRowStride % 4 == 0 && kWaveLds % 4 == 0, "rows must stay 16-byte aligned");
-#define SLT_LDS __attribute__((address_space(3)))
 
 struct AdjOut {
     float* __restrict__ adj;
@@ -237,26 +235,26 @@ __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, 
     // per cell (scalar loop): its member lanes' ids in a list, their count, the cell id
     uint64_t pending = __ballot(need);
     int nslots = 0;
-    while (pending) {
+    while (pending && nslots < kFlushSlots) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const int key = __builtin_amdgcn_readlane(M.id, leader);
         const bool mine = need && M.id == key;
         const uint64_t mem = __ballot(mine);
         pending &= ~mem;
-        if (nslots < kFlushSlots) {
-            if (mine) {
-                const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(mem >> 32),
-                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mem, 0));
-                reinterpret_cast<SLT_LDS uint8_t*>(out.lists() + nslots * 16)[pos] = (uint8_t)lane;
-            }
-            if (lane == 0) {
-                out.ids()[nslots] = key;
-                out.counts()[nslots] = __popcll(mem);
-            }
-            ++nslots;
-        } else if (mine) {
-            scatter_corners(G, out.adj, M.id, M);
+        if (mine) {
+            const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(mem >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mem, 0));
+            reinterpret_cast<SLT_LDS uint8_t*>(out.lists() + nslots * 16)[pos] = (uint8_t)lane;
         }
+        if (lane == 0) {
+            out.ids()[nslots] = key;
+            out.counts()[nslots] = __popcll(mem);
+        }
+        ++nslots;
+    }
+    // lanes of cells beyond the slots scatter their own moments from their rows
+    if (pending) {
+        if ((pending >> lane) & 1) scatter_row(G, out.adj, M.id, out.lds + lane * kRowStride);
     }
     wave_lds_sync();
     // per (slot, channel, monomial S) entry (lane-parallel), the eight monomials of a
