@@ -453,13 +453,15 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
                 s.dv[a] = v.dv[a][t];
                 s.v0[a] = v0[a];
             }
-            s.tau = v.tau[t];
+            s.tau = 0.0;
             s.fold = v.fold[t];
             s.path = 0.0;
             s.nout = 0;
             const int64_t g0 = (int64_t)c * A.K;
             const int64_t g1 = g0 + A.K < A.sched ? g0 + A.K : A.sched;
-            integrate<KAPPA, false, GlobalTape>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1,
+            // the tape needs the trajectory only, which kappa does not change: the
+            // replay skips the kappa channel and tau
+            integrate<false, false, GlobalTape>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1,
                                                 none, tape);
         }
         for (int round = 0; __ballot(round < tape.count); ++round) {
