@@ -288,6 +288,27 @@ __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, 
         atomicAdd(out.adj + (int64_t)(out.ids()[sl] + off) * 4 + ch, acc);
     }
     wave_lds_sync();
+    // the lanes that gave their moments start their next cell from zero
+    float4* mom[8] = {&M.m0, &M.mx, &M.my, &M.mz, &M.mxy, &M.mxz, &M.myz, &M.mxyz};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        mom[q]->x = need ? 0.f : mom[q]->x;
+        mom[q]->y = need ? 0.f : mom[q]->y;
+        mom[q]->z = need ? 0.f : mom[q]->z;
+        mom[q]->w = need ? 0.f : mom[q]->w;
+    }
+}
+
+// Makes the cache hold cell `id` for every lane with a point in the box (in);
+// when any lane of the wavefront needs a new cell, every lane reloads (a lane
+// without a point reloads its cached cell, or voxel 0 with id -1), so the old
+// polynomial is dead where the new one is written.
+__device__ __forceinline__ void fetch_cell(const DevGrid& G, bool in, int id, Cell& cc) {
+    const int want = in ? id : cc.id;
+    if (__ballot(want != cc.id)) {
+        load_cell(G, want < 0 ? 0 : want, cc);
+        cc.id = want;
+    }
 }
 
 // Adds one stage's adjoint Ab at cell `id`, local coordinates t, to the lane's
@@ -298,11 +319,7 @@ __device__ __forceinline__ void accumulate(const DevGrid& G, const AdjOut& out, 
                                            const float4& Ab) {
     const bool move = contrib && id != M.id;
     if (__ballot(move && M.id >= 0)) flush_wave(G, out, M, move && M.id >= 0);
-    if (move) {
-        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = z4;
-        M.id = id;
-    }
+    M.id = move ? id : M.id;
     if (contrib) {
         const float txy = tx * ty, txz = tx * tz, tyz = ty * tz, txyz = txy * tz;
         M.m0 = f4add(M.m0, Ab);
@@ -326,18 +343,15 @@ __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float u0, f
     const bool in = act && in_box(G, u0, u1, u2);
     int id = -1;
     float tx = 0.f, ty = 0.f, tz = 0.f;
+    const int i0 = min((int)u0, G.n0 - 2);
+    const int i1 = min((int)u1, G.n1 - 2);
+    const int i2 = min((int)u2, G.n2 - 2);
+    if (in) id = (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2));
+    fetch_cell(G, in, id, cc);
     if (in) {
-        const int i0 = min((int)u0, G.n0 - 2);
-        const int i1 = min((int)u1, G.n1 - 2);
-        const int i2 = min((int)u2, G.n2 - 2);
         tx = u0 - (float)i0;
         ty = u1 - (float)i1;
         tz = u2 - (float)i2;
-        id = (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2));
-        if (id != cc.id) {
-            load_cell(G, id, cc);
-            cc.id = id;
-        }
         // channel-contracted coefficients: Kq = sum_ch Ab_ch * kq_ch
         auto dot = [&](const float4& k) {
             return fmaf(Ab.w, k.w, fmaf(Ab.z, k.z, fmaf(Ab.y, k.y, Ab.x * k.x)));
@@ -350,6 +364,20 @@ __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float u0, f
         rb[2] = fmaf(ty, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxz, Kz));
     }
     accumulate(G, out, M, in, id, tx, ty, tz, Ab);
+}
+
+// Forward RHS (gradient channels) at the grid-unit point u for the lanes with a
+// step (act); zero outside the box.  Collective (uniform cell reloads).
+__device__ __forceinline__ void fwd_eval(const DevGrid& G, bool act, float u0, float u1, float u2,
+                                         Cell& cc, float4& A) {
+    const bool in = act && in_box(G, u0, u1, u2);
+    const int i0 = min((int)u0, G.n0 - 2);
+    const int i1 = min((int)u1, G.n1 - 2);
+    const int i2 = min((int)u2, G.n2 - 2);
+    const int id = in ? (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2)) : -1;
+    fetch_cell(G, in, id, cc);
+    A = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (in) poly_eval<false>(cc, u0 - (float)i0, u1 - (float)i1, u2 - (float)i2, A);
 }
 
 // Transposed RK4 step.  On entry rb, vb are the adjoints of the state after the
@@ -366,11 +394,9 @@ __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float
     const float hh6 = h * h6;
     // forward stage points (the same arithmetic as integrate())
     float4 A1 = make_float4(0.f, 0.f, 0.f, 0.f), A2 = A1;
-    if (act) {
-        if (in_box(G, rh[0], rh[1], rh[2])) eval_rhs<KAPPA, false>(G, rh[0], rh[1], rh[2], cc, A1);
-        eval_rhs<KAPPA, true>(G, fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]),
-                              fmaf(hh, vh[2], rh[2]), cc, A2);
-    }
+    fwd_eval(G, act, rh[0], rh[1], rh[2], cc, A1);
+    fwd_eval(G, act, fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]), fmaf(hh, vh[2], rh[2]), cc,
+             A2);
     const float r2[3] = {fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]), fmaf(hh, vh[2], rh[2])};
     const float v2[3] = {fmaf(hh, A1.x, vh[0]), fmaf(hh, A1.y, vh[1]), fmaf(hh, A1.z, vh[2])};
     const float r3[3] = {fmaf(hh, v2[0], rh[0]), fmaf(hh, v2[1], rh[1]), fmaf(hh, v2[2], rh[2])};
@@ -397,14 +423,13 @@ __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float
                                   fmaf(hh, v2b[1], fmaf(hh6, rb[1], h6 * vb[1])),
                                   fmaf(hh, v2b[2], fmaf(hh6, rb[2], h6 * vb[2])), h6 * tk);
     eval_adj(G, act, rh[0], rh[1], rh[2], B1, cc, M, out, p1);
-    if (act) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            // v: r2 = r + h/2 v, v2, v3, r' = r + h v + ..., v' = v + ...
-            const float nv = vb[c] + h * rb[c] + v3b[c] + v2b[c] + hh * p2[c];
-            rb[c] = rb[c] + p1[c] + p2[c] + p3[c] + p4[c];
-            vb[c] = nv;
-        }
+    for (int c = 0; c < 3; ++c) {
+        // v: r2 = r + h/2 v, v2, v3, r' = r + h v + ..., v' = v + ...
+        const float nv = vb[c] + h * rb[c] + v3b[c] + v2b[c] + hh * p2[c];
+        const float nr = rb[c] + p1[c] + p2[c] + p3[c] + p4[c];
+        rb[c] = act ? nr : rb[c];
+        vb[c] = act ? nv : vb[c];
     }
 }
 
@@ -440,6 +465,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
     cc.id = -1;
     Moments M;
     M.id = -1;
+    M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = make_float4(0.f, 0.f, 0.f, 0.f);
     const SaveSink none{nullptr, nullptr, n, ray, 0, A.nseg, A.s0 + 3 * n, n, ray};
     for (int c = A.nck - 1; c >= 0; --c) {
         GlobalTape tape{A.tape, n, tt, 0};
@@ -465,24 +491,24 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
                                                 none, tape);
         }
         for (int round = 0; __ballot(round < tape.count); ++round) {
-            const bool act = round < tape.count;
-            float rh[3] = {0.f, 0.f, 0.f}, vh[3] = {0.f, 0.f, 0.f}, hT = 0.f;
-            if (act) {
-                const float* p = tape.slot(tape.count - 1 - round);
-                hT = p[6 * n + tt];
+            // every lane reads a tape slot (a lane whose tape is done reads slot 0 of its
+            // row, inside the tape, and uses none of it); a straight move's slot holds
+            // only its duration
+            const bool live = round < tape.count;
+            const float* p = tape.slot(live ? tape.count - 1 - round : 0);
+            const float hT = p[6 * n + tt];
+            float rh[3], vh[3];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    rh[a] = p[a * n + tt];
-                    vh[a] = p[(3 + a) * n + tt];
-                }
+            for (int a = 0; a < 3; ++a) {
+                rh[a] = p[a * n + tt];
+                vh[a] = p[(3 + a) * n + tt];
             }
-            if (act && hT < 0.f) {
-                // straight move r' = r + T v
+            // straight move r' = r + T v
+            const float T = live && hT < 0.f ? -hT : 0.f;
 #pragma unroll
-                for (int a = 0; a < 3; ++a) vb[a] = fmaf(-hT, rb[a], vb[a]);
-            }
-            step_adj<KAPPA>(A.G, act && hT > 0.f, rh, vh, hT > 0.f ? hT : 0.f, rb, vb, tb, cc,
-                            M, out);
+            for (int a = 0; a < 3; ++a) vb[a] = fmaf(T, rb[a], vb[a]);
+            const bool act = live && hT > 0.f;
+            step_adj<KAPPA>(A.G, act, rh, vh, act ? hT : 0.f, rb, vb, tb, cc, M, out);
         }
     }
     flush_wave(A.G, out, M, M.id >= 0);
