@@ -413,8 +413,8 @@ int slt_adjoint(const void* packed, const slt_grid_desc* g, const slt_adjoint_ar
  * hipMalloc).  The reverse pass's tape lives in a library-owned device array
  * of SLT_ADJ_SCRATCH_BYTES, one per GPU, held from its first use until the
  * process ends; a->tape must be NULL.  Limit: slt_tape_bytes(n_rays,
- * ckpt_every) <= SLT_ADJ_SCRATCH_BYTES, i.e. n_rays * ckpt_every <= 4793490
- * (e.g. 299593 rays at K = 16); larger calls return SLT_ERR_UNSUPPORTED
+ * ckpt_every) <= SLT_ADJ_SCRATCH_BYTES, i.e. n_rays * ckpt_every <= 4194304
+ * (e.g. 262144 rays at K = 16); larger calls return SLT_ERR_UNSUPPORTED
  * before any device work.  The checkpoints, seeds and grid_adj stay
  * caller-owned, as in slt_adjoint, and the result is the same.  Every call on
  * one GPU shares that GPU's array, so calls that overlap on one GPU (other

@@ -54,26 +54,28 @@ __device__ __forceinline__ CkptView ckpt_view(void* base, int64_t n, int c) {
     return v;
 }
 
-// Tape: per slot (one move), SoA over rays: rh[3], vh[3], hT (h > 0 for an RK4
-// step, -T for a straight move of duration T).
+// Tape: per slot (one move), one 32-byte record per ray, the rays of a slot
+// contiguous: (rh[3], hT) and (vh[3], unused), hT = h > 0 for an RK4 step, -T for
+// a straight move of duration T (whose record holds only hT).  A record is
+// written and read as two 16-byte vectors.
+constexpr int64_t kTapeRecBytes = 32;
+
 struct GlobalTape {
     float* base;
     int64_t n;
     int64_t t;
     int count;
-    __device__ __forceinline__ float* slot(int j) const { return base + (int64_t)j * 7 * n; }
+    __device__ __forceinline__ slt_f4* rec(int j) const {
+        return reinterpret_cast<slt_f4*>(base) + ((int64_t)j * n + t) * 2;
+    }
     __device__ __forceinline__ void rk(const float* rh, const float* vh, float h) {
-        float* p = slot(count++);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            p[c * n + t] = rh[c];
-            p[(3 + c) * n + t] = vh[c];
-        }
-        p[6 * n + t] = h;
+        slt_f4* p = rec(count++);
+        p[0] = slt_f4{rh[0], rh[1], rh[2], h};
+        p[1] = slt_f4{vh[0], vh[1], vh[2], 0.f};
     }
     __device__ __forceinline__ void straight(float T) {
-        float* p = slot(count++);
-        p[6 * n + t] = -T;
+        slt_f4* p = rec(count++);
+        p[0] = slt_f4{0.f, 0.f, 0.f, -T};
     }
 };
 
@@ -491,18 +493,14 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
                                                 none, tape);
         }
         for (int round = 0; __ballot(round < tape.count); ++round) {
-            // every lane reads a tape slot (a lane whose tape is done reads slot 0 of its
-            // row, inside the tape, and uses none of it); a straight move's slot holds
-            // only its duration
+            // every lane reads a tape record (a lane whose tape is done reads its ray's
+            // record of slot 0, inside the tape, and uses none of it); a straight move's
+            // record holds only its duration
             const bool live = round < tape.count;
-            const float* p = tape.slot(live ? tape.count - 1 - round : 0);
-            const float hT = p[6 * n + tt];
-            float rh[3], vh[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                rh[a] = p[a * n + tt];
-                vh[a] = p[(3 + a) * n + tt];
-            }
+            const slt_f4* p = tape.rec(live ? tape.count - 1 - round : 0);
+            const slt_f4 q0 = p[0], q1 = p[1];
+            const float hT = q0.w;
+            const float rh[3] = {q0.x, q0.y, q0.z}, vh[3] = {q1.x, q1.y, q1.z};
             // straight move r' = r + T v
             const float T = live && hT < 0.f ? -hT : 0.f;
 #pragma unroll
@@ -653,7 +651,7 @@ extern "C" int64_t slt_checkpoint_bytes(int64_t n_rays, int64_t sched_steps, int
 
 extern "C" int64_t slt_tape_bytes(int64_t n_rays, int32_t ckpt_every) {
     if (n_rays < 0 || ckpt_every < 1) return 0;
-    return (int64_t)ckpt_every * 7 * 4 * n_rays;
+    return (int64_t)ckpt_every * kTapeRecBytes * n_rays;
 }
 
 extern "C" int slt_forward_ckpt(const void* packed, const slt_grid_desc* g,
@@ -758,9 +756,9 @@ extern "C" int slt_adjoint_noalloc(const void* packed, const slt_grid_desc* g,
         set_error("slt_adjoint_noalloc: seed_state and grid_adj are required");
         return SLT_ERR_INVALID;
     }
-    // the tape holds 28 bytes per ray and step of an interval: n * K <= cap, tested
+    // the tape holds one record per ray and step of an interval: n * K <= cap, tested
     // without forming the product (it could overflow)
-    const int64_t cap = SLT_ADJ_SCRATCH_BYTES / 28;
+    const int64_t cap = SLT_ADJ_SCRATCH_BYTES / kTapeRecBytes;
     if (A.n > cap / A.K) {
         set_error("slt_adjoint_noalloc: %lld rays at ckpt_every %d exceed the %lld-byte tape "
                   "scratch (n_rays * ckpt_every <= %lld)",

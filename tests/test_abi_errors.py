@@ -318,9 +318,9 @@ def test_sizes_of_invalid_requests_are_zero(L):
     assert L.slt_checkpoint_bytes(10, 10, 0) == 0
     assert L.slt_tape_bytes(10, 0) == 0
     assert L.slt_ray_state_bytes(10) > 0
-    # 28 bytes per ray and recorded step; at most a fixed number of steps are recorded
-    assert L.slt_tape_bytes(10, 1) == 28 * 10
-    assert L.slt_tape_bytes(10, 16) % (28 * 10) == 0 and L.slt_tape_bytes(10, 16) <= 16 * 28 * 10
+    # one 32-byte record per ray and recorded step; at most a fixed number of steps are recorded
+    assert L.slt_tape_bytes(10, 1) == 32 * 10
+    assert L.slt_tape_bytes(10, 16) % (32 * 10) == 0 and L.slt_tape_bytes(10, 16) <= 16 * 32 * 10
 
 
 def test_valu_calibrate_rejections(L):
