@@ -186,11 +186,11 @@ __device__ __forceinline__ void scatter_row(const DevGrid& G, float* __restrict_
 // Where flushed moments go: the voxel adjoint in global memory and this
 // wavefront's flush area in LDS: one row of 32 moments per lane (8 float4: m0,
 // mx, my, mz, mxy, mxz, myz, mxyz; rows 16-byte aligned), then kFlushSlots
-// cell ids, their member counts and their member lists (up to 64 lane ids of one
-// byte each per slot).
+// cell ids, their member counts, their member lists (up to 64 lane ids of one
+// byte each per slot) and their corner adjoints (8 float4 per slot).
 constexpr int kFlushSlots = 16;
 constexpr int kRowStride = 36;
-constexpr int kWaveLds = 64 * kRowStride + 2 * kFlushSlots + kFlushSlots * 16;
+constexpr int kWaveLds = 64 * kRowStride + 2 * kFlushSlots + kFlushSlots * 16 + kFlushSlots * 32;
 static_assert(kRowStride % 4 == 0 && kWaveLds % 4 == 0, "rows must stay 16-byte aligned");
 
 struct AdjOut {
@@ -202,6 +202,9 @@ struct AdjOut {
     __device__ __forceinline__ SLT_LDS int* counts() const { return ids() + kFlushSlots; }
     __device__ __forceinline__ SLT_LDS uint32_t* lists() const {
         return reinterpret_cast<SLT_LDS uint32_t*>(counts() + kFlushSlots);
+    }
+    __device__ __forceinline__ SLT_LDS slt_f4* corners() const {
+        return reinterpret_cast<SLT_LDS slt_f4*>(lists() + kFlushSlots * 16);
     }
 };
 
@@ -216,14 +219,15 @@ __device__ __forceinline__ void wave_lds_sync() {
 // `need` set give their moments to their cell.  A sorted wavefront holds a few
 // cells at a time, and at a flush nearly all its lanes leave theirs together.
 // The flushing lanes store their moments in their LDS rows; a scalar loop over
-// the cells gives each cell a slot and lists its member lanes there; then every
-// lane sums the member rows of one (slot, channel, monomial) entry at a time, four
-// member reads in flight (a fixed order per wavefront), turns the eight monomial
-// sums of its (slot, channel) into the eight corner adjoints with three butterfly
-// steps across the lanes that hold them, and adds its corner's value to the voxel
-// adjoint: a flush of c cells issues ceil(32 c / 64) full-width atomic
-// instructions.  No LDS atomics: float atomics on LDS addresses shared by many
-// lanes serialise.  Lanes of cells beyond the slots scatter their own.
+// the cells gives each cell a slot and lists its member lanes there; then each
+// lane sums the member rows of one (slot, monomial) entry, all four channels per
+// 16-byte read, four member reads in flight (a fixed order per wavefront), and
+// the eight lanes of a slot turn their monomial sums into the eight corner
+// adjoints with three butterfly steps.  The corners go through LDS so that each
+// atomic instruction adds the four channels of 16 corners: a flush of c cells
+// issues ceil(32 c / 64) full-width atomic instructions whose lanes share voxels.
+// No LDS atomics: float atomics on LDS addresses shared by many lanes serialise.
+// Lanes of cells beyond the slots scatter their own.
 __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, Moments& M,
                                            bool need) {
     const int lane = __lane_id();
@@ -259,35 +263,45 @@ __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, 
         if ((pending >> lane) & 1) scatter_row(G, out.adj, M.id, out.lds + lane * kRowStride);
     }
     wave_lds_sync();
-    // per (slot, channel, monomial S) entry (lane-parallel), the eight monomials of a
-    // channel in eight adjacent lanes: the sum over the slot's member rows, four member
-    // reads in flight; then the corner adjoints by the superset Moebius transform across
-    // those lanes (corner E = sum over S containing E of (-1)^|S \ E| m_S), one butterfly
-    // per axis bit; each lane adds its corner's value to the voxel adjoint.
-    for (int e = lane; e < nslots * 32; e += 64) {
-        const int sl = e >> 5, ch = (e >> 3) & 3, S = e & 7;
+    // per (slot, monomial S) entry (lane-parallel), the eight monomials of a slot in
+    // eight adjacent lanes: the sum over the slot's member rows of moment q(S), all four
+    // channels in one 16-byte read per member, four member reads in flight; then the
+    // corner adjoints by the superset Moebius transform across those lanes (corner E =
+    // sum over S containing E of (-1)^|S \ E| m_S), one butterfly per axis bit; each lane
+    // adds its corner's four channels to the voxel adjoint.
+    const SLT_LDS slt_f4* rows = reinterpret_cast<const SLT_LDS slt_f4*>(out.lds);
+    for (int e = lane; e < nslots * 8; e += 64) {
+        const int sl = e >> 3, S = e & 7;
         const int q = (0x74516230 >> (4 * S)) & 7;  // the moment of monomial set S
-        const int k = 4 * q + ch;
         const int cnt = out.counts()[sl];
         const SLT_LDS uint32_t* lst = out.lists() + sl * 16;
-        float acc = 0.f;
+        slt_f4 acc = {0.f, 0.f, 0.f, 0.f};
         for (int i = 0; i < cnt; i += 4) {
             const uint32_t w = lst[i >> 2];
-            float v[4];
+            slt_f4 v[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                v[r] = out.lds[((w >> (8 * r)) & 63u) * kRowStride + k];
-                v[r] = i + r < cnt ? v[r] : 0.f;
+                v[r] = rows[((w >> (8 * r)) & 63u) * (kRowStride / 4) + q];
+                v[r] = i + r < cnt ? v[r] : slt_f4{0.f, 0.f, 0.f, 0.f};
             }
             acc += (v[0] + v[1]) + (v[2] + v[3]);
         }
 #pragma unroll
         for (int bit = 1; bit < 8; bit <<= 1) {
-            const float partner = __shfl_xor(acc, bit);
+            const slt_f4 partner = {__shfl_xor(acc.x, bit), __shfl_xor(acc.y, bit),
+                                    __shfl_xor(acc.z, bit), __shfl_xor(acc.w, bit)};
             acc = (S & bit) ? acc : acc - partner;
         }
+        out.corners()[e] = acc;
+    }
+    wave_lds_sync();
+    // each lane adds one channel of one corner; one instruction covers the 4 channels of
+    // 16 corners, so the atomics on a voxel's channels travel together
+    for (int e = lane; e < nslots * 32; e += 64) {
+        const int sl = e >> 5, ch = (e >> 3) & 3, S = e & 7;
         const int off = ((S >> 2) & 1) * G.s0 + ((S >> 1) & 1) * G.s1 + (S & 1) * G.s2;
-        atomicAdd(out.adj + (int64_t)(out.ids()[sl] + off) * 4 + ch, acc);
+        atomicAdd(out.adj + (int64_t)(out.ids()[sl] + off) * 4 + ch,
+                  reinterpret_cast<const SLT_LDS float*>(out.corners())[(sl * 8 + S) * 4 + ch]);
     }
     wave_lds_sync();
     // the lanes that gave their moments start their next cell from zero
