@@ -315,16 +315,56 @@ __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, 
     }
 }
 
-// Makes the cache hold cell `id` for every lane with a point in the box (in);
-// when any lane of the wavefront needs a new cell, every lane reloads (a lane
-// without a point reloads its cached cell, or voxel 0 with id -1), so the old
+// The reverse steps' cell cache: polynomial, voxel index of corner (i, j, k)
+// (-1 = none), that corner in grid units (NaN = none) and whether the cell
+// touches a face of the grid.
+struct AdjCell : CellPoly {
+    int id;
+    float c0, c1, c2;
+    bool face;
+};
+
+// Locates the grid-unit point u of a lane with a step (act): returns whether it
+// lies in the closed box, and its local coordinates t in the cell the cache then
+// holds (floor(u), clamped to the last cell on each axis, as the forward finds
+// it).  A point in the cached cell, when that cell is interior, is inside and
+// needs nothing more (t = u - corner is exact there).  When any lane of the
+// wavefront has another point, every lane takes the full test together, and if
+// any lane needs a new cell every lane reloads its cell (a lane that keeps its
+// cell reloads it; one with no cell loads voxel 0 and keeps id -1), so the old
 // polynomial is dead where the new one is written.
-__device__ __forceinline__ void fetch_cell(const DevGrid& G, bool in, int id, Cell& cc) {
-    const int want = in ? id : cc.id;
-    if (__ballot(want != cc.id)) {
-        load_cell(G, want < 0 ? 0 : want, cc);
-        cc.id = want;
+__device__ __forceinline__ bool locate(const DevGrid& G, bool act, float u0, float u1, float u2,
+                                       AdjCell& cc, float& tx, float& ty, float& tz) {
+    tx = u0 - cc.c0;
+    ty = u1 - cc.c1;
+    tz = u2 - cc.c2;
+    const bool fast = (unit_interval(tx) & unit_interval(ty) & unit_interval(tz)) && !cc.face;
+    bool in = act;
+    if (__ballot(act && !fast)) {
+        const bool inb = in_box(G, u0, u1, u2);
+        in = act && (fast || inb);
+        const bool moving = act && !fast && inb;
+        const int i0 = min((int)u0, G.n0 - 2);
+        const int i1 = min((int)u1, G.n1 - 2);
+        const int i2 = min((int)u2, G.n2 - 2);
+        const int want =
+            moving ? (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2)) : cc.id;
+        if (__ballot(want != cc.id)) {
+            load_cell(G, want < 0 ? 0 : want, cc);
+            cc.id = want;
+        }
+        if (moving) {
+            cc.c0 = (float)i0;
+            cc.c1 = (float)i1;
+            cc.c2 = (float)i2;
+            cc.face = (min(min(i0, i1), i2) == 0) | (i0 == G.n0 - 2) | (i1 == G.n1 - 2) |
+                      (i2 == G.n2 - 2);
+            tx = u0 - cc.c0;
+            ty = u1 - cc.c1;
+            tz = u2 - cc.c2;
+        }
     }
+    return in;
 }
 
 // Adds one stage's adjoint Ab at cell `id`, local coordinates t, to the lane's
@@ -353,21 +393,12 @@ __device__ __forceinline__ void accumulate(const DevGrid& G, const AdjOut& out, 
 // (gx, gy, gz, kappa): returns d(loss)/d(u) in rb and accumulates the grid
 // adjoint.  Every lane of the wavefront calls it together; `act` lanes contribute.
 __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float u0, float u1, float u2,
-                                         const float4& Ab, Cell& cc, Moments& M,
+                                         const float4& Ab, AdjCell& cc, Moments& M,
                                          const AdjOut& out, float rb[3]) {
     rb[0] = rb[1] = rb[2] = 0.f;
-    const bool in = act && in_box(G, u0, u1, u2);
-    int id = -1;
-    float tx = 0.f, ty = 0.f, tz = 0.f;
-    const int i0 = min((int)u0, G.n0 - 2);
-    const int i1 = min((int)u1, G.n1 - 2);
-    const int i2 = min((int)u2, G.n2 - 2);
-    if (in) id = (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2));
-    fetch_cell(G, in, id, cc);
+    float tx, ty, tz;
+    const bool in = locate(G, act, u0, u1, u2, cc, tx, ty, tz);
     if (in) {
-        tx = u0 - (float)i0;
-        ty = u1 - (float)i1;
-        tz = u2 - (float)i2;
         // channel-contracted coefficients: Kq = sum_ch Ab_ch * kq_ch
         auto dot = [&](const float4& k) {
             return fmaf(Ab.w, k.w, fmaf(Ab.z, k.z, fmaf(Ab.y, k.y, Ab.x * k.x)));
@@ -379,21 +410,17 @@ __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float u0, f
         rb[1] = fmaf(tz, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxy, Ky));
         rb[2] = fmaf(ty, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxz, Kz));
     }
-    accumulate(G, out, M, in, id, tx, ty, tz, Ab);
+    accumulate(G, out, M, in, cc.id, in ? tx : 0.f, in ? ty : 0.f, in ? tz : 0.f, Ab);
 }
 
 // Forward RHS (gradient channels) at the grid-unit point u for the lanes with a
 // step (act); zero outside the box.  Collective (uniform cell reloads).
 __device__ __forceinline__ void fwd_eval(const DevGrid& G, bool act, float u0, float u1, float u2,
-                                         Cell& cc, float4& A) {
-    const bool in = act && in_box(G, u0, u1, u2);
-    const int i0 = min((int)u0, G.n0 - 2);
-    const int i1 = min((int)u1, G.n1 - 2);
-    const int i2 = min((int)u2, G.n2 - 2);
-    const int id = in ? (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2)) : -1;
-    fetch_cell(G, in, id, cc);
+                                         AdjCell& cc, float4& A) {
+    float tx, ty, tz;
+    const bool in = locate(G, act, u0, u1, u2, cc, tx, ty, tz);
     A = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (in) poly_eval<false>(cc, u0 - (float)i0, u1 - (float)i1, u2 - (float)i2, A);
+    if (in) poly_eval<false>(cc, tx, ty, tz, A);
 }
 
 // Transposed RK4 step.  On entry rb, vb are the adjoints of the state after the
@@ -402,7 +429,7 @@ __device__ __forceinline__ void fwd_eval(const DevGrid& G, bool act, float u0, f
 template <bool KAPPA>
 __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float rh[3],
                                          const float vh[3], float h, float rb[3], float vb[3],
-                                         float tb, Cell& cc, Moments& M,
+                                         float tb, AdjCell& cc, Moments& M,
                                          const AdjOut& out) {
     const float hh = 0.5f * h;
     const float h6 = h * (1.0f / 6.0f);
@@ -477,8 +504,10 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
         }
         tb = A.seed_tau ? (float)A.seed_tau[ray] : 0.f;
     }
-    Cell cc;
+    AdjCell cc;
     cc.id = -1;
+    cc.c0 = cc.c1 = cc.c2 = __builtin_nanf("");
+    cc.face = true;
     Moments M;
     M.id = -1;
     M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = make_float4(0.f, 0.f, 0.f, 0.f);

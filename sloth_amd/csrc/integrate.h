@@ -14,11 +14,6 @@ struct CellPoly {
     float4 k0, kx, ky, kz, kxy, kxz, kyz, kxyz;
 };
 
-// Cell cached by the adjoint's reverse steps: polynomial + voxel index.
-struct Cell : CellPoly {
-    int id;  // voxel index of corner (i, j, k); -1 = none cached
-};
-
 // Cell cached by the forward integration: polynomial + the cell's lower
 // corner in grid units (i, j, k as floats; NaN = none cached) + whether the
 // cell touches a face of the grid.  A stage point whose local coordinates
@@ -213,26 +208,6 @@ __device__ __forceinline__ bool eval_rhs(const DevGrid& G, float u0, float u1, f
     poly_eval<KAPPA>(cc, tx, ty, tz, out);
     if (CHECK && __builtin_expect(!inside, 0)) out = make_float4(0.f, 0.f, 0.f, 0.f);
     return inside;
-}
-
-// The same with the adjoint's cell cache (keyed by voxel index).
-template <bool KAPPA, bool CHECK>
-__device__ __forceinline__ bool eval_rhs(const DevGrid& G, float u0, float u1, float u2, Cell& cc,
-                                         float4& out) {
-    if (CHECK && !in_box(G, u0, u1, u2)) {
-        out = make_float4(0.f, 0.f, 0.f, 0.f);
-        return false;
-    }
-    const int i0 = min((int)u0, G.n0 - 2);
-    const int i1 = min((int)u1, G.n1 - 2);
-    const int i2 = min((int)u2, G.n2 - 2);
-    const int id = (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2));
-    if (id != cc.id) {
-        load_cell(G, id, cc);
-        cc.id = id;
-    }
-    poly_eval<KAPPA>(cc, u0 - (float)i0, u1 - (float)i1, u2 - (float)i2, out);
-    return true;
 }
 
 // Number of whole steps of length h (at most kmax) that a ray outside the box
