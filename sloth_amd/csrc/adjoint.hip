@@ -12,8 +12,9 @@
 //    step over the tape in reverse.  Each stage re-evaluates the cached cell's
 //    trilinear polynomial; its Jacobian gives the state adjoint, and the
 //    stage's share of the grid adjoint is accumulated per lane as the 8
-//    polynomial moments of the current cell and flushed to global memory
-//    (atomics, one float4 per corner) only when the cell changes.
+//    polynomial moments of the current cell (in registers within a step, in
+//    the lane's LDS row between steps) and flushed to global memory (atomics,
+//    combined across the lanes of a cell) only when the cell changes.
 //  * slt_adjoint_unpack turns the voxel adjoint (d/dx, d/dy, d/dz of coeff ne,
 //    kappa) into dL/dn_e: the transposed numpy.gradient stencil times coeff,
 //    plus d kappa / d n_e times the kappa adjoint.
@@ -136,7 +137,8 @@ forward_ckpt_kernel(AdjArgs A) {
 }
 
 // Per-lane grid-adjoint accumulator: the 8 polynomial moments (per channel) of
-// the stage adjoints that fell into one cell.
+// the stage adjoints of this round that fell into the lane's cell; the moments
+// of earlier rounds in the same cell are in the lane's LDS row.
 struct Moments {
     int id;  // cell (voxel index of corner (i, j, k)); -1 = empty
     float4 m0, mx, my, mz, mxy, mxz, myz, mxyz;
@@ -208,6 +210,14 @@ struct AdjOut {
     }
 };
 
+// Adds a lane's moment sums to its LDS row.
+__device__ __forceinline__ void add_to_row(const AdjOut& out, const Moments& M) {
+    SLT_LDS slt_f4* row = reinterpret_cast<SLT_LDS slt_f4*>(out.lds + __lane_id() * kRowStride);
+    const float4* src[8] = {&M.m0, &M.mx, &M.my, &M.mz, &M.mxy, &M.mxz, &M.myz, &M.mxyz};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) row[q] += slt_f4{src[q]->x, src[q]->y, src[q]->z, src[q]->w};
+}
+
 // Orders this wavefront's LDS accesses across its lanes.
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -218,8 +228,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Collective flush (whole wavefront, uniform control flow): the lanes with
 // `need` set give their moments to their cell.  A sorted wavefront holds a few
 // cells at a time, and at a flush nearly all its lanes leave theirs together.
-// The flushing lanes store their moments in their LDS rows; a scalar loop over
-// the cells gives each cell a slot and lists its member lanes there; then each
+// Every lane adds its round's sums to its LDS row (which holds its moments of
+// the current cell), so no moment stays in registers across the flush; a
+// scalar loop over the flushing lanes' cells gives each cell a slot and lists
+// its member lanes there; then each
 // lane sums the member rows of one (slot, monomial) entry, all four channels per
 // 16-byte read, four member reads in flight (a fixed order per wavefront), and
 // the eight lanes of a slot turn their monomial sums into the eight corner
@@ -227,16 +239,15 @@ __device__ __forceinline__ void wave_lds_sync() {
 // atomic instruction adds the four channels of 16 corners: a flush of c cells
 // issues ceil(32 c / 64) full-width atomic instructions whose lanes share voxels.
 // No LDS atomics: float atomics on LDS addresses shared by many lanes serialise.
-// Lanes of cells beyond the slots scatter their own.
+// Lanes of cells beyond the slots scatter their own.  The flushing lanes' rows
+// are then cleared for their next cells.
 __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, Moments& M,
                                            bool need) {
     const int lane = __lane_id();
-    if (need) {
-        SLT_LDS slt_f4* row = reinterpret_cast<SLT_LDS slt_f4*>(out.lds + lane * kRowStride);
-        const float4* src[8] = {&M.m0, &M.mx, &M.my, &M.mz, &M.mxy, &M.mxz, &M.myz, &M.mxyz};
-#pragma unroll
-        for (int q = 0; q < 8; ++q) row[q] = slt_f4{src[q]->x, src[q]->y, src[q]->z, src[q]->w};
-    }
+    // every lane with a cell moves its sums to its row, so no moment is held in
+    // registers across the flush
+    if (M.id >= 0) add_to_row(out, M);
+    M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = make_float4(0.f, 0.f, 0.f, 0.f);
     wave_lds_sync();
     // per cell (scalar loop): its member lanes' ids in a list, their count, the cell id
     uint64_t pending = __ballot(need);
@@ -303,16 +314,13 @@ __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, 
         atomicAdd(out.adj + (int64_t)(out.ids()[sl] + off) * 4 + ch,
                   reinterpret_cast<const SLT_LDS float*>(out.corners())[(sl * 8 + S) * 4 + ch]);
     }
-    wave_lds_sync();
     // the lanes that gave their moments start their next cell from zero
-    float4* mom[8] = {&M.m0, &M.mx, &M.my, &M.mz, &M.mxy, &M.mxz, &M.myz, &M.mxyz};
+    if (need) {
+        SLT_LDS slt_f4* row = reinterpret_cast<SLT_LDS slt_f4*>(out.lds + lane * kRowStride);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        mom[q]->x = need ? 0.f : mom[q]->x;
-        mom[q]->y = need ? 0.f : mom[q]->y;
-        mom[q]->z = need ? 0.f : mom[q]->z;
-        mom[q]->w = need ? 0.f : mom[q]->w;
+        for (int q = 0; q < 8; ++q) row[q] = slt_f4{0.f, 0.f, 0.f, 0.f};
     }
+    wave_lds_sync();
 }
 
 // The reverse steps' cell cache: polynomial, voxel index of corner (i, j, k)
@@ -478,8 +486,13 @@ __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float
 
 // One ray per lane; the lanes of a wavefront walk their tapes in lockstep
 // rounds so that grid-adjoint flushes can be combined across the wavefront.
+// Three wavefronts per SIMD: nothing larger than a step's values is carried from
+// round to round in registers (the moments go to the LDS rows, the cell
+// polynomial is reloaded), so the kernel fits 168 VGPRs with a few spills, and
+// 12.4 KiB of LDS per wavefront lets three 256-lane blocks share a CU.
 template <bool KAPPA>
-__global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+adjoint_kernel(AdjArgs A) {
     const int lb = xcd_block(blockIdx.x, gridDim.x, A.nblocks);
     if (lb < 0) return;  // whole block: padding of the XCD remap
     const int64_t t = (int64_t)lb * blockDim.x + threadIdx.x;
@@ -511,6 +524,13 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
     Moments M;
     M.id = -1;
     M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = make_float4(0.f, 0.f, 0.f, 0.f);
+    {
+        // the lane's moments of its current cell live in its LDS row between rounds
+        SLT_LDS slt_f4* row =
+            reinterpret_cast<SLT_LDS slt_f4*>(out.lds + __lane_id() * kRowStride);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) row[q] = slt_f4{0.f, 0.f, 0.f, 0.f};
+    }
     const SaveSink none{nullptr, nullptr, n, ray, 0, A.nseg, A.s0 + 3 * n, n, ray};
     for (int c = A.nck - 1; c >= 0; --c) {
         GlobalTape tape{A.tape, n, tt, 0};
@@ -549,7 +569,14 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjArgs A) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) vb[a] = fmaf(T, rb[a], vb[a]);
             const bool act = live && hT > 0.f;
+            // the cell polynomial is not carried from round to round: every lane reloads its
+            // cached cell (voxel 0 while it has none)
+            load_cell(A.G, cc.id < 0 ? 0 : cc.id, cc);
             step_adj<KAPPA>(A.G, act, rh, vh, act ? hT : 0.f, rb, vb, tb, cc, M, out);
+            // the round's sums go to the row; the next round starts from zero
+            if (M.id >= 0) add_to_row(out, M);
+            M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz =
+                make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
     flush_wave(A.G, out, M, M.id >= 0);
