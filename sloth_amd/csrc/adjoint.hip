@@ -241,13 +241,17 @@ __device__ __forceinline__ void wave_lds_sync() {
 // No LDS atomics: float atomics on LDS addresses shared by many lanes serialise.
 // Lanes of cells beyond the slots scatter their own.  The flushing lanes' rows
 // are then cleared for their next cells.
+template <bool SUMS = true>
 __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, Moments& M,
                                            bool need) {
     const int lane = __lane_id();
     // every lane with a cell moves its sums to its row, so no moment is held in
     // registers across the flush
-    if (M.id >= 0) add_to_row(out, M);
-    M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (SUMS) {
+        if (M.id >= 0) add_to_row(out, M);
+        M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz =
+            make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     wave_lds_sync();
     // per cell (scalar loop): its member lanes' ids in a list, their count, the cell id
     uint64_t pending = __ballot(need);
@@ -378,13 +382,34 @@ __device__ __forceinline__ bool locate(const DevGrid& G, bool act, float u0, flo
 // Adds one stage's adjoint Ab at cell `id`, local coordinates t, to the lane's
 // moments (contrib = this lane has a contribution).  Called by every lane of
 // the wavefront together.
+template <bool FIRST>
 __device__ __forceinline__ void accumulate(const DevGrid& G, const AdjOut& out, Moments& M,
                                            bool contrib, int id, float tx, float ty, float tz,
                                            const float4& Ab) {
     const bool move = contrib && id != M.id;
-    if (__ballot(move && M.id >= 0)) flush_wave(G, out, M, move && M.id >= 0);
+    // (the round's first stage holds no sums yet: they went to the rows at the end of
+    // the previous round)
+    if (__ballot(move && M.id >= 0)) flush_wave<!FIRST>(G, out, M, move && M.id >= 0);
     M.id = move ? id : M.id;
-    if (contrib) {
+    if (FIRST) {
+        // the round's first stage: the sums are zero here (cleared at the end of the
+        // previous round and by any flush), so it writes them (Ab = 0 without a
+        // contribution)
+        const float4 A0 = make_float4(contrib ? Ab.x : 0.f, contrib ? Ab.y : 0.f,
+                                      contrib ? Ab.z : 0.f, contrib ? Ab.w : 0.f);
+        const float txy = tx * ty, txz = tx * tz, tyz = ty * tz, txyz = txy * tz;
+        auto mul = [](float s, const float4& a) {
+            return make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
+        };
+        M.m0 = A0;
+        M.mx = mul(tx, A0);
+        M.my = mul(ty, A0);
+        M.mz = mul(tz, A0);
+        M.mxy = mul(txy, A0);
+        M.mxz = mul(txz, A0);
+        M.myz = mul(tyz, A0);
+        M.mxyz = mul(txyz, A0);
+    } else if (contrib) {
         const float txy = tx * ty, txz = tx * tz, tyz = ty * tz, txyz = txy * tz;
         M.m0 = f4add(M.m0, Ab);
         f4fma(M.mx, tx, Ab);
@@ -400,6 +425,7 @@ __device__ __forceinline__ void accumulate(const DevGrid& G, const AdjOut& out, 
 // Adjoint of one RHS evaluation at the grid-unit point u with output adjoint Ab
 // (gx, gy, gz, kappa): returns d(loss)/d(u) in rb and accumulates the grid
 // adjoint.  Every lane of the wavefront calls it together; `act` lanes contribute.
+template <bool FIRST>
 __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float u0, float u1, float u2,
                                          const float4& Ab, AdjCell& cc, Moments& M,
                                          const AdjOut& out, float rb[3]) {
@@ -418,7 +444,7 @@ __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float u0, f
         rb[1] = fmaf(tz, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxy, Ky));
         rb[2] = fmaf(ty, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxz, Kz));
     }
-    accumulate(G, out, M, in, cc.id, in ? tx : 0.f, in ? ty : 0.f, in ? tz : 0.f, Ab);
+    accumulate<FIRST>(G, out, M, in, cc.id, in ? tx : 0.f, in ? ty : 0.f, in ? tz : 0.f, Ab);
 }
 
 // Forward RHS (gradient channels) at the grid-unit point u for the lanes with a
@@ -457,23 +483,23 @@ __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float
     float p4[3], p3[3], p2[3], p1[3];
     // stage 4: a4 enters v' with weight h/6
     const float4 B4 = make_float4(h6 * vb[0], h6 * vb[1], h6 * vb[2], h6 * tk);
-    eval_adj(G, act, r4[0], r4[1], r4[2], B4, cc, M, out, p4);
+    eval_adj<true>(G, act, r4[0], r4[1], r4[2], B4, cc, M, out, p4);
     const float v3b[3] = {h * p4[0], h * p4[1], h * p4[2]};  // r4 = r + h v3
     // stage 3: a3 enters r' (h^2/6) and v' (h/3)
     const float4 B3 = make_float4(fmaf(hh6, rb[0], h3 * vb[0]), fmaf(hh6, rb[1], h3 * vb[1]),
                                   fmaf(hh6, rb[2], h3 * vb[2]), h3 * tk);
-    eval_adj(G, act, r3[0], r3[1], r3[2], B3, cc, M, out, p3);
+    eval_adj<false>(G, act, r3[0], r3[1], r3[2], B3, cc, M, out, p3);
     const float v2b[3] = {hh * p3[0], hh * p3[1], hh * p3[2]};  // r3 = r + h/2 v2
     // stage 2: a2 enters r' (h^2/6), v' (h/3) and v3 = v + h/2 a2
     const float4 B2 = make_float4(fmaf(hh, v3b[0], fmaf(hh6, rb[0], h3 * vb[0])),
                                   fmaf(hh, v3b[1], fmaf(hh6, rb[1], h3 * vb[1])),
                                   fmaf(hh, v3b[2], fmaf(hh6, rb[2], h3 * vb[2])), h3 * tk);
-    eval_adj(G, act, r2[0], r2[1], r2[2], B2, cc, M, out, p2);
+    eval_adj<false>(G, act, r2[0], r2[1], r2[2], B2, cc, M, out, p2);
     // stage 1: a1 enters r' (h^2/6), v' (h/6) and v2 = v + h/2 a1
     const float4 B1 = make_float4(fmaf(hh, v2b[0], fmaf(hh6, rb[0], h6 * vb[0])),
                                   fmaf(hh, v2b[1], fmaf(hh6, rb[1], h6 * vb[1])),
                                   fmaf(hh, v2b[2], fmaf(hh6, rb[2], h6 * vb[2])), h6 * tk);
-    eval_adj(G, act, rh[0], rh[1], rh[2], B1, cc, M, out, p1);
+    eval_adj<false>(G, act, rh[0], rh[1], rh[2], B1, cc, M, out, p1);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         // v: r2 = r + h/2 v, v2, v3, r' = r + h v + ..., v' = v + ...
@@ -573,13 +599,11 @@ adjoint_kernel(AdjArgs A) {
             // cached cell (voxel 0 while it has none)
             load_cell(A.G, cc.id < 0 ? 0 : cc.id, cc);
             step_adj<KAPPA>(A.G, act, rh, vh, act ? hT : 0.f, rb, vb, tb, cc, M, out);
-            // the round's sums go to the row; the next round starts from zero
+            // the round's sums go to the row; the next round's first stage rewrites them
             if (M.id >= 0) add_to_row(out, M);
-            M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz =
-                make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
-    flush_wave(A.G, out, M, M.id >= 0);
+    flush_wave<false>(A.G, out, M, M.id >= 0);  // every sum is in the rows by now
 }
 
 // ---------------------------------------------------------------------------
