@@ -353,6 +353,10 @@ __device__ __forceinline__ bool locate(const DevGrid& G, bool act, float u0, flo
     const bool fast = (unit_interval(tx) & unit_interval(ty) & unit_interval(tz)) && !cc.face;
     bool in = act;
     if (__ballot(act && !fast)) {
+        // the full test runs only in this branch: an empty asm statement hands the point
+        // over inside it, so the compiler does not compute the box test and the cell index
+        // ahead of the branch for every call
+        asm volatile("" : "+v"(u0), "+v"(u1), "+v"(u2));
         const bool inb = in_box(G, u0, u1, u2);
         in = act && (fast || inb);
         const bool moving = act && !fast && inb;
