@@ -187,19 +187,20 @@ __device__ __forceinline__ void scatter_row(const DevGrid& G, float* __restrict_
 
 // Where flushed moments go: the voxel adjoint in global memory and this
 // wavefront's flush area in LDS: one row of 32 moments per lane (8 float4: m0,
-// mx, my, mz, mxy, mxz, myz, mxyz; rows 16-byte aligned), then kFlushSlots
-// cell ids, their member counts, their member lists (up to 64 lane ids of one
-// byte each per slot) and their corner adjoints (8 float4 per slot).
+// mx, my, mz, mxy, mxz, myz, mxyz; rows 16-byte aligned) and a 65th row of
+// zeros, then kFlushSlots cell ids, their member counts, their member lists (up
+// to 64 row numbers of one byte each per slot) and their corner adjoints (8
+// float4 per slot).
 constexpr int kFlushSlots = 16;
 constexpr int kRowStride = 36;
-constexpr int kWaveLds = 64 * kRowStride + 2 * kFlushSlots + kFlushSlots * 16 + kFlushSlots * 32;
+constexpr int kWaveLds = 65 * kRowStride + 2 * kFlushSlots + kFlushSlots * 16 + kFlushSlots * 32;
 static_assert(kRowStride % 4 == 0 && kWaveLds % 4 == 0, "rows must stay 16-byte aligned");
 
 struct AdjOut {
     float* __restrict__ adj;
     SLT_LDS float* lds;
     __device__ __forceinline__ SLT_LDS int* ids() const {
-        return reinterpret_cast<SLT_LDS int*>(lds + 64 * kRowStride);
+        return reinterpret_cast<SLT_LDS int*>(lds + 65 * kRowStride);
     }
     __device__ __forceinline__ SLT_LDS int* counts() const { return ids() + kFlushSlots; }
     __device__ __forceinline__ SLT_LDS uint32_t* lists() const {
@@ -253,6 +254,11 @@ __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, 
             make_float4(0.f, 0.f, 0.f, 0.f);
     }
     wave_lds_sync();
+    // member lists start out naming row 64, which is all zeros, so a sum that reads a
+    // whole word of four entries past a list's end adds nothing
+    reinterpret_cast<SLT_LDS slt_f4*>(out.lists())[lane] =
+        __builtin_bit_cast(slt_f4, make_uint4(0x40404040u, 0x40404040u, 0x40404040u, 0x40404040u));
+    wave_lds_sync();
     // per cell (scalar loop): its member lanes' ids in a list, their count, the cell id
     uint64_t pending = __ballot(need);
     int nslots = 0;
@@ -295,10 +301,7 @@ __device__ __forceinline__ void flush_wave(const DevGrid& G, const AdjOut& out, 
             const uint32_t w = lst[i >> 2];
             slt_f4 v[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                v[r] = rows[((w >> (8 * r)) & 63u) * (kRowStride / 4) + q];
-                v[r] = i + r < cnt ? v[r] : slt_f4{0.f, 0.f, 0.f, 0.f};
-            }
+            for (int r = 0; r < 4; ++r) v[r] = rows[((w >> (8 * r)) & 127u) * (kRowStride / 4) + q];
             acc += (v[0] + v[1]) + (v[2] + v[3]);
         }
 #pragma unroll
@@ -555,11 +558,15 @@ adjoint_kernel(AdjArgs A) {
     M.id = -1;
     M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = make_float4(0.f, 0.f, 0.f, 0.f);
     {
-        // the lane's moments of its current cell live in its LDS row between rounds
+        // the lane's moments of its current cell live in its LDS row between rounds; row 64
+        // stays zero (the member lists' filler)
         SLT_LDS slt_f4* row =
             reinterpret_cast<SLT_LDS slt_f4*>(out.lds + __lane_id() * kRowStride);
 #pragma unroll
         for (int q = 0; q < 8; ++q) row[q] = slt_f4{0.f, 0.f, 0.f, 0.f};
+        if (__lane_id() < kRowStride / 4)
+            reinterpret_cast<SLT_LDS slt_f4*>(out.lds + 64 * kRowStride)[__lane_id()] =
+                slt_f4{0.f, 0.f, 0.f, 0.f};
     }
     const SaveSink none{nullptr, nullptr, n, ray, 0, A.nseg, A.s0 + 3 * n, n, ray};
     for (int c = A.nck - 1; c >= 0; --c) {
