@@ -259,8 +259,9 @@ def serving_bench(n_requests):
 
 def adjoint_bench(n_rays=2_000_000, n=256):
     """Config A: 256^3 Kolmogorov (10 mm), 2e6 rays at 532 nm with absorption; loss = MSE of
-    detector positions and energies against a trace through a second seeded grid;
-    checkpoints every 16 steps.  Times the checkpointed forward and the backward."""
+    detector positions and energies against a trace through a second seeded grid.  Times
+    the forward and the backward with checkpoints every 16 steps, then with the whole
+    trajectory recorded by the forward (ckpt_every = 0, no replay in the backward)."""
     import torch
 
     from sloth_amd.adjoint import detector_loss, trace_states
@@ -273,31 +274,45 @@ def adjoint_bench(n_rays=2_000_000, n=256):
     tne, _ = kolmogorov_field(n, L, n0=2e24, frac=0.3, l_min=0.2e-3, seed=22, device=dev)
     s0 = torch.from_numpy(Beam(n_rays, 4e-3, 0.0, L / 2, beam_type="square",
                                seeded=True).s0).to(dev)
-    kw = dict(lwl=532e-9, inv_brems=True, Te=20.0, Z=1.0, coulomb_log=10.0, ckpt_every=16)
+    kw = dict(lwl=532e-9, inv_brems=True, Te=20.0, Z=1.0, coulomb_log=10.0)
     with torch.no_grad():
         ts, tt, _ = trace_states(torch.from_numpy(tne).to(dev), s0, *coords, **kw)
     tp, te = ts[0:2].double(), torch.exp(-tt)
+    del ts, tt
     ne_t = torch.from_numpy(ne).to(dev)
-    times = []
-    for rep in range(3):
-        x = ne_t.clone().requires_grad_(True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        st, tau, total = trace_states(x, s0, *coords, **kw)
-        loss = detector_loss(st, tau, tp, te, pos_scale=1e-4)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        loss.backward()
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        times.append((t1 - t0, t2 - t1))
-    fwd, bwd = min(t[0] for t in times), min(t[1] for t in times)
-    steps = int(total.item())
-    log(f"[adjoint] forward {fwd * 1e3:.1f} ms, backward {bwd * 1e3:.1f} ms, {steps} ray-steps")
-    return {"workload": "kolmogorov_256_2e6_rays_absorption", "ray_steps": steps,
-            "forward_ckpt_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
-            "backward_ray_steps_per_s": steps / bwd, "ckpt_every": 16,
-            "grad_norm": float(x.grad.double().norm())}
+    res = {"workload": "kolmogorov_256_2e6_rays_absorption"}
+    for K in (16, 0):
+        times = []
+        for rep in range(3):
+            x = ne_t.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st, tau, total = trace_states(x, s0, *coords, ckpt_every=K, **kw)
+            loss = detector_loss(st, tau, tp, te, pos_scale=1e-4)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            loss.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            times.append((t1 - t0, t2 - t1))
+            del st, tau, loss
+        fwd, bwd = min(t[0] for t in times), min(t[1] for t in times)
+        steps = int(total.item())
+        log(f"[adjoint] ckpt_every {K}: forward {fwd * 1e3:.1f} ms, backward {bwd * 1e3:.1f} ms, "
+            f"{steps} ray-steps")
+        leg = {"forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
+               "backward_ray_steps_per_s": steps / bwd, "ckpt_every": K,
+               "grad_norm": float(x.grad.double().norm())}
+        if K:
+            res.update(ray_steps=steps, forward_ckpt_ms=leg["forward_ms"],
+                       backward_ms=leg["backward_ms"],
+                       backward_ray_steps_per_s=leg["backward_ray_steps_per_s"], ckpt_every=K,
+                       grad_norm=leg["grad_norm"])
+        else:
+            res["trajectory_tape"] = leg
+        del x, total
+        torch.cuda.empty_cache()
+    return res
 
 
 def main():

@@ -377,7 +377,8 @@ typedef struct slt_adjoint_args {
     const float* s0;              /* 6 x n_rays start states (caller order)              */
     const int32_t* order;         /* processing order (Morton) or NULL                    */
     int32_t n_seg;                /* save schedule, as in slt_trace_args                  */
-    int32_t ckpt_every;           /* K: checkpoint every K schedule steps (e.g. 16)       */
+    int32_t ckpt_every;           /* K: checkpoint every K schedule steps (e.g. 16); 0:
+                                     no checkpoints, the forward records the trajectory */
     const int32_t* seg_nsub;
     const float* seg_h;
     int64_t sched_steps;          /* sum of seg_nsub                                      */
@@ -387,14 +388,27 @@ typedef struct slt_adjoint_args {
     const float* seed_state;      /* backward in: d loss / d final state, 6 x n_rays      */
     const double* seed_tau;       /* backward in: d loss / d final tau, or NULL           */
     float* grid_adj;              /* backward out (accumulated): d loss / d voxel, packed layout, 4 floats per voxel */
-    void* tape;                   /* backward scratch: slt_tape_bytes(n_rays, K) bytes    */
+    void* tape;                   /* backward scratch: slt_tape_bytes(n_rays, K) bytes;
+                                     K = 0: the trajectory tape, slt_trajectory_tape_bytes
+                                     (n_rays, sched_steps) bytes, written by slt_forward_ckpt
+                                     and read by slt_adjoint (keep it between the two)   */
     unsigned long long* total_steps; /* forward: += in-box steps, or NULL                 */
 } slt_adjoint_args;
 
 int64_t slt_checkpoint_bytes(int64_t n_rays, int64_t sched_steps, int32_t ckpt_every);
 int64_t slt_tape_bytes(int64_t n_rays, int32_t ckpt_every);
+/*
+ * Bytes of the whole-trajectory tape of ckpt_every = 0: one 32-byte record per
+ * ray and schedule step at most (sched_steps <= INT32_MAX; 0 when the size does
+ * not fit int64).  It trades memory for the reverse pass's replay: 2e6 rays over
+ * 1443 steps take 92 GB, with no checkpoint replayed.  The checkpoint area is
+ * then slt_checkpoint_bytes(n_rays, sched_steps, 0) = 4 * n_rays bytes (each
+ * ray's record count).  No counterpart in the reference.
+ */
+int64_t slt_trajectory_tape_bytes(int64_t n_rays, int64_t sched_steps);
 
-/* Forward trace (as slt_trace) that also stores every ray's state each K steps. */
+/* Forward trace (as slt_trace) that also stores every ray's state each K steps
+ * (K = 0: records every move on a->tape instead). */
 int slt_forward_ckpt(const void* packed, const slt_grid_desc* g, const slt_adjoint_args* a,
                      slt_stream_t stream);
 
@@ -402,6 +416,7 @@ int slt_forward_ckpt(const void* packed, const slt_grid_desc* g, const slt_adjoi
  * Reverse pass: from the seeds (adjoints of the final states and tau), replay
  * each K-step interval from its checkpoint and run the transposed RK4 steps,
  * accumulating d loss / d voxel into grid_adj (zero it first; float atomics).
+ * K = 0: no replay; the transposed steps walk the forward's trajectory tape.
  */
 int slt_adjoint(const void* packed, const slt_grid_desc* g, const slt_adjoint_args* a,
                 slt_stream_t stream);
@@ -414,8 +429,8 @@ int slt_adjoint(const void* packed, const slt_grid_desc* g, const slt_adjoint_ar
  * of SLT_ADJ_SCRATCH_BYTES, one per GPU, held from its first use until the
  * process ends; a->tape must be NULL.  Limit: slt_tape_bytes(n_rays,
  * ckpt_every) <= SLT_ADJ_SCRATCH_BYTES, i.e. n_rays * ckpt_every <= 4194304
- * (e.g. 262144 rays at K = 16); larger calls return SLT_ERR_UNSUPPORTED
- * before any device work.  The checkpoints, seeds and grid_adj stay
+ * (e.g. 262144 rays at K = 16); larger calls, and K = 0 (whose tape the
+ * caller holds), return SLT_ERR_UNSUPPORTED before any device work.  The checkpoints, seeds and grid_adj stay
  * caller-owned, as in slt_adjoint, and the result is the same.  Every call on
  * one GPU shares that GPU's array, so calls that overlap on one GPU (other
  * streams, other threads) race and must be serialised by the caller.  The

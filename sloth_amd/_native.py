@@ -25,7 +25,7 @@ EXPORTED = (
     "slt_ray_state_bytes", "slt_ray_state_init", "slt_job_table_prepare", "slt_advance",
     "slt_checkpoint_bytes", "slt_tape_bytes", "slt_forward_ckpt", "slt_adjoint",
     "slt_adjoint_unpack", "slt_rtm", "slt_beam", "slt_adjoint_noalloc", "slt_valu_calibrate",
-    "slt_ray_work_bytes", "slt_gather_rays", "slt_unpermute_rays",
+    "slt_ray_work_bytes", "slt_gather_rays", "slt_unpermute_rays", "slt_trajectory_tape_bytes",
 )
 
 SLT_MAX_JOBS = 64
@@ -190,6 +190,8 @@ def lib():
     L.slt_checkpoint_bytes.restype = i64
     L.slt_tape_bytes.argtypes = [i64, i32]
     L.slt_tape_bytes.restype = i64
+    L.slt_trajectory_tape_bytes.argtypes = [i64, i64]
+    L.slt_trajectory_tape_bytes.restype = i64
     L.slt_forward_ckpt.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(AdjointArgs), vp]
     L.slt_adjoint.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(AdjointArgs), vp]
     L.slt_adjoint_noalloc.argtypes = [vp, ctypes.POINTER(GridDesc), ctypes.POINTER(AdjointArgs),

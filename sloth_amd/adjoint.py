@@ -6,7 +6,9 @@ any torch loss built from them flow back to ``ne`` through the hand-written
 adjoint kernels (``slt_forward_ckpt`` stores the ray state every
 ``ckpt_every`` steps, ``slt_adjoint`` runs the transposed RK4 steps interval by
 interval, ``slt_adjoint_unpack`` applies the transposed gradient stencil and
-d kappa / d n_e).  The gradient is that of the discrete fixed-step RK4 map
+d kappa / d n_e).  ``ckpt_every=0`` trades memory for time: the forward records
+every move of the trajectory (32 bytes per ray and step, held until the
+backward) and the backward replays nothing.  The gradient is that of the discrete fixed-step RK4 map
 (the one the forward computes), so it matches central finite differences of
 the same trace.
 """
@@ -72,6 +74,12 @@ class _TraceFn(torch.autograd.Function):
         h_d = engine.const_device(h.astype(np.float32), dev)
         ck = torch.empty(max(int(L.slt_checkpoint_bytes(n, sched, K)), 1), dtype=torch.uint8,
                          device=dev)
+        traj = None
+        if K == 0:
+            nb = int(L.slt_trajectory_tape_bytes(n, sched))
+            if n > 0 and sched > 0 and nb == 0:
+                raise ValueError(f"a trajectory tape of {n} rays x {sched} steps is too large")
+            traj = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
         state = torch.empty((6, n), dtype=torch.float32, device=dev)
         tau = torch.zeros(n, dtype=torch.float64, device=dev)
         total = torch.zeros(1, dtype=torch.int64, device=dev)
@@ -79,11 +87,12 @@ class _TraceFn(torch.autograd.Function):
         a.final_state = state.data_ptr()
         a.final_tau = tau.data_ptr()
         a.total_steps = total.data_ptr()
+        a.tape = N.ptr(traj)
         N.check(L.slt_forward_ckpt(grid.voxels.data_ptr(), ctypes.byref(grid.desc),
                                    ctypes.byref(a), N.stream_handle()))
         ctx.cfg = cfg
         ctx.grid = grid
-        ctx.saved = (ne_d, rays, order, nsub_d, h_d, ck, sched, K, total)
+        ctx.saved = (ne_d, rays, order, nsub_d, h_d, ck, sched, K, total, traj)
         ctx.mark_non_differentiable(total)
         return state, tau, total
 
@@ -91,7 +100,7 @@ class _TraceFn(torch.autograd.Function):
     def backward(ctx, g_state, g_tau, _g_total):
         cfg = ctx.cfg
         grid = ctx.grid
-        ne_d, rays, order, nsub_d, h_d, ck, sched, K, _ = ctx.saved
+        ne_d, rays, order, nsub_d, h_d, ck, sched, K, _, traj = ctx.saved
         dev = ne_d.device
         L = N.lib()
         n = rays.shape[1]
@@ -100,7 +109,8 @@ class _TraceFn(torch.autograd.Function):
         seed = g_state.to(torch.float32).contiguous()
         seed_tau = g_tau.to(torch.float64).contiguous() if g_tau is not None else None
         adj = torch.zeros(grid.nbytes // 4, dtype=torch.float32, device=dev)
-        tape = torch.empty(max(int(L.slt_tape_bytes(n, K)), 1), dtype=torch.uint8, device=dev)
+        tape = traj if K == 0 else torch.empty(max(int(L.slt_tape_bytes(n, K)), 1),
+                                               dtype=torch.uint8, device=dev)
         a = _args(grid, rays, order, nsub_d, h_d, sched, ck, K)
         a.seed_state = seed.data_ptr()
         a.seed_tau = N.ptr(seed_tau)
@@ -140,7 +150,15 @@ def trace_states(ne: torch.Tensor, s0, x, y, z, *, lwl: float = 532e-9,
     the reference solve horizon sqrt(8) L / c with L the grid extent along the
     probing axis.  Returns (state (6, N) float32, tau (N,) float64, total
     in-box ray-steps (1,) int64).
+
+    ``ckpt_every`` K > 0 stores each ray's state every K steps (60 bytes per ray
+    and checkpoint) and replays each interval in the backward.  K = 0 stores the
+    whole trajectory instead (32 bytes per ray and schedule step, e.g. 92 GB for
+    2e6 rays over 1443 steps, held from the forward to the backward) and the
+    backward replays nothing.
     """
+    if int(ckpt_every) < 0:
+        raise ValueError(f"ckpt_every must be >= 0, got {ckpt_every}")
     N.require_device()
     coords = tuple(engine.host_f32(c).reshape(-1) for c in (x, y, z))
     ax = engine.axis_index(probing_direction)

@@ -15,6 +15,10 @@
 //    polynomial moments of the current cell (in registers within a step, in
 //    the lane's LDS row between steps) and flushed to global memory (atomics,
 //    combined across the lanes of a cell) only when the cell changes.
+//  * With ckpt_every = 0 there are no checkpoints: slt_forward_ckpt records
+//    every move of the whole trajectory onto the caller's tape (n_rays x
+//    sched_steps records at most, memory traded for the replay), and
+//    slt_adjoint reads that tape in reverse without replaying anything.
 //  * slt_adjoint_unpack turns the voxel adjoint (d/dx, d/dy, d/dz of coeff ne,
 //    kappa) into dL/dn_e: the transposed numpy.gradient stencil times coeff,
 //    plus d kappa / d n_e times the kappa adjoint.
@@ -105,7 +109,9 @@ struct AdjArgs {
 // Five wavefronts per SIMD, as the offline trace kernel: at 96 VGPRs the
 // values that spill are touched around the save points and in the rare
 // no-stage-in-the-box branch, not in the step loop (7.9 vs 8.3 ms at four).
-template <bool KAPPA>
+// TAPE (ckpt_every = 0): no checkpoints; every move of the whole trajectory
+// goes onto the tape and each ray's record count to the checkpoint area.
+template <bool KAPPA, bool TAPE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 forward_ckpt_kernel(AdjArgs A) {
     const int lb = xcd_block(blockIdx.x, gridDim.x, A.nblocks);
@@ -117,7 +123,16 @@ forward_ckpt_kernel(AdjArgs A) {
         RayState s;
         state_from_rays(A.G, A.s0, A.n, ray, s);
         SaveSink sink{A.final_state, A.final_tau, A.n, ray, 0, A.nseg, A.s0 + 3 * A.n, A.n, ray};
-        for (int c = 0; c < A.nck; ++c) {
+        if (TAPE) {
+            // plain stores: a load's wait also waits for the stores before it, and
+            // non-temporal stores are acknowledged later (15.7 vs 11.5 ms on
+            // 256^3 / 2e6 rays)
+            GlobalTape tape{A.tape, A.n, t, 0};
+            integrate<KAPPA, false, GlobalTape>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, 0, A.sched,
+                                                sink, tape);
+            static_cast<int32_t*>(A.ckpt)[t] = tape.count;
+        }
+        for (int c = 0; !TAPE && c < A.nck; ++c) {
             const CkptView v = ckpt_view(A.ckpt, A.n, c);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -523,7 +538,9 @@ __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float
 // round to round in registers (the moments go to the LDS rows, the cell
 // polynomial is reloaded), so the kernel fits 168 VGPRs with a few spills, and
 // 12.4 KiB of LDS per wavefront lets three 256-lane blocks share a CU.
-template <bool KAPPA>
+// REPLAY = false (ckpt_every = 0): the forward pass recorded the whole
+// trajectory, so the one interval's tape is read as it is, with no replay.
+template <bool KAPPA, bool REPLAY>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 adjoint_kernel(AdjArgs A) {
     const int lb = xcd_block(blockIdx.x, gridDim.x, A.nblocks);
@@ -571,7 +588,8 @@ adjoint_kernel(AdjArgs A) {
     const SaveSink none{nullptr, nullptr, n, ray, 0, A.nseg, A.s0 + 3 * n, n, ray};
     for (int c = A.nck - 1; c >= 0; --c) {
         GlobalTape tape{A.tape, n, tt, 0};
-        if (valid) {
+        if (!REPLAY) tape.count = valid ? static_cast<const int32_t*>(A.ckpt)[t] : 0;
+        if (REPLAY && valid) {
             const CkptView v = ckpt_view(A.ckpt, n, c);
             RayState s;
 #pragma unroll
@@ -711,8 +729,8 @@ static int fill_adj_args(const void* packed, const slt_grid_desc* g, const slt_a
                          AdjArgs* A, const char* who) {
     int rc = make_dev_grid(packed, g, &A->G);
     if (rc) return rc;
-    if (!a || a->n_rays < 0 || a->n_seg < 1 || a->ckpt_every < 1 || !a->seg_nsub || !a->seg_h ||
-        (a->n_rays > 0 && (!a->s0 || !a->checkpoints))) {
+    if (!a || a->n_rays < 0 || a->n_seg < 1 || a->ckpt_every < 0 || a->sched_steps < 0 ||
+        !a->seg_nsub || !a->seg_h || (a->n_rays > 0 && (!a->s0 || !a->checkpoints))) {
         set_error("%s: invalid arguments", who);
         return SLT_ERR_INVALID;
     }
@@ -722,8 +740,13 @@ static int fill_adj_args(const void* packed, const slt_grid_desc* g, const slt_a
     A->nseg = a->n_seg;
     A->K = a->ckpt_every;
     A->sched = a->sched_steps;
-    A->nck = (int)((a->sched_steps + a->ckpt_every - 1) / a->ckpt_every);
+    // ckpt_every = 0: one interval, the whole trajectory on the forward's tape
+    A->nck = a->ckpt_every ? (int)((a->sched_steps + a->ckpt_every - 1) / a->ckpt_every) : 1;
     if (A->nck < 1) A->nck = 1;
+    if (a->ckpt_every == 0 && a->sched_steps > INT32_MAX) {
+        set_error("%s: a trajectory tape holds at most %d moves per ray", who, INT32_MAX);
+        return SLT_ERR_UNSUPPORTED;
+    }
     A->seg_nsub = a->seg_nsub;
     A->seg_h = a->seg_h;
     A->ckpt = a->checkpoints;
@@ -748,7 +771,8 @@ static int fill_adj_args(const void* packed, const slt_grid_desc* g, const slt_a
 using namespace slt;
 
 extern "C" int64_t slt_checkpoint_bytes(int64_t n_rays, int64_t sched_steps, int32_t ckpt_every) {
-    if (n_rays < 0 || sched_steps < 0 || ckpt_every < 1) return 0;
+    if (n_rays < 0 || sched_steps < 0 || ckpt_every < 0) return 0;
+    if (ckpt_every == 0) return 4 * n_rays;  // each ray's tape record count
     int64_t nck = (sched_steps + ckpt_every - 1) / ckpt_every;
     if (nck < 1) nck = 1;
     return nck * kCkptBytesPerRay * n_rays;
@@ -757,6 +781,13 @@ extern "C" int64_t slt_checkpoint_bytes(int64_t n_rays, int64_t sched_steps, int
 extern "C" int64_t slt_tape_bytes(int64_t n_rays, int32_t ckpt_every) {
     if (n_rays < 0 || ckpt_every < 1) return 0;
     return (int64_t)ckpt_every * kTapeRecBytes * n_rays;
+}
+
+extern "C" int64_t slt_trajectory_tape_bytes(int64_t n_rays, int64_t sched_steps) {
+    if (n_rays < 0 || sched_steps < 0 || sched_steps > INT32_MAX) return 0;
+    // one record per ray and schedule step at most (a straight move covers several)
+    if (n_rays > 0 && sched_steps > INT64_MAX / kTapeRecBytes / n_rays) return 0;
+    return sched_steps * kTapeRecBytes * n_rays;
 }
 
 extern "C" int slt_forward_ckpt(const void* packed, const slt_grid_desc* g,
@@ -770,13 +801,18 @@ extern "C" int slt_forward_ckpt(const void* packed, const slt_grid_desc* g,
         set_error("slt_forward_ckpt: final_state is required");
         return SLT_ERR_INVALID;
     }
+    const bool traj = a->ckpt_every == 0;
+    if (traj && !A.tape) {
+        set_error("slt_forward_ckpt: ckpt_every = 0 records the trajectory on a->tape "
+                  "(slt_trajectory_tape_bytes bytes), which is NULL");
+        return SLT_ERR_INVALID;
+    }
     const int grid = xcd_pad(A.nblocks);
-    if (g->nchan >= 4)
-        hipLaunchKernelGGL(forward_ckpt_kernel<true>, dim3(grid), dim3(256), 0,
-                           (hipStream_t)stream, A);
-    else
-        hipLaunchKernelGGL(forward_ckpt_kernel<false>, dim3(grid), dim3(256), 0,
-                           (hipStream_t)stream, A);
+    auto* kern = g->nchan >= 4 ? (traj ? forward_ckpt_kernel<true, true>
+                                       : forward_ckpt_kernel<true, false>)
+                               : (traj ? forward_ckpt_kernel<false, true>
+                                       : forward_ckpt_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, (hipStream_t)stream, A);
     return check_launch("slt_forward_ckpt");
 }
 
@@ -791,12 +827,12 @@ extern "C" int slt_adjoint(const void* packed, const slt_grid_desc* g, const slt
         set_error("slt_adjoint: seed_state, grid_adj and tape are required");
         return SLT_ERR_INVALID;
     }
+    const bool replay = a->ckpt_every > 0;
     const int grid = xcd_pad(A.nblocks);
-    if (g->nchan >= 4)
-        hipLaunchKernelGGL(adjoint_kernel<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, A);
-    else
-        hipLaunchKernelGGL(adjoint_kernel<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream,
-                           A);
+    auto* kern = g->nchan >= 4
+                     ? (replay ? adjoint_kernel<true, true> : adjoint_kernel<true, false>)
+                     : (replay ? adjoint_kernel<false, true> : adjoint_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, (hipStream_t)stream, A);
     return check_launch("slt_adjoint");
 }
 
@@ -856,6 +892,11 @@ extern "C" int slt_adjoint_noalloc(const void* packed, const slt_grid_desc* g,
         set_error("slt_adjoint_noalloc: tape must be NULL (the tape is the library's scratch)");
         return SLT_ERR_INVALID;
     }
+    if (a->ckpt_every == 0) {
+        set_error("slt_adjoint_noalloc: ckpt_every = 0 needs the forward's trajectory tape; "
+                  "call slt_adjoint with it");
+        return SLT_ERR_UNSUPPORTED;
+    }
     if (A.n == 0) return SLT_OK;
     if (!A.seed_state || !A.grid_adj) {
         set_error("slt_adjoint_noalloc: seed_state and grid_adj are required");
@@ -878,10 +919,11 @@ extern "C" int slt_adjoint_noalloc(const void* packed, const slt_grid_desc* g,
     A.tape = static_cast<float*>(tape);
     const int grid = xcd_pad(A.nblocks);
     if (g->nchan >= 4)
-        hipLaunchKernelGGL(adjoint_kernel<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, A);
+        hipLaunchKernelGGL((adjoint_kernel<true, true>), dim3(grid), dim3(256), 0,
+                           (hipStream_t)stream, A);
     else
-        hipLaunchKernelGGL(adjoint_kernel<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream,
-                           A);
+        hipLaunchKernelGGL((adjoint_kernel<false, true>), dim3(grid), dim3(256), 0,
+                           (hipStream_t)stream, A);
     return check_launch("slt_adjoint_noalloc");
 }
 

@@ -231,8 +231,8 @@ def test_adjoint_rejections(L):
         return a
 
     for fn, name in ((L.slt_forward_ckpt, "slt_forward_ckpt"), (L.slt_adjoint, "slt_adjoint")):
-        for bad in (dict(ckpt_every=0), dict(n_seg=0), dict(checkpoints=None), dict(s0=None),
-                    dict(n_rays=-1), dict(seg_h=None)):
+        for bad in (dict(ckpt_every=-1), dict(n_seg=0), dict(checkpoints=None), dict(s0=None),
+                    dict(n_rays=-1), dict(seg_h=None), dict(sched_steps=-1)):
             _expect(L, fn(FAKE, ctypes.byref(g), ctypes.byref(aargs(**bad)), None),
                     N.SLT_ERR_INVALID, name)
         _expect(L, fn(None, ctypes.byref(g), ctypes.byref(aargs()), None), N.SLT_ERR_INVALID,
@@ -245,6 +245,18 @@ def test_adjoint_rejections(L):
     for missing in ("seed_state", "grid_adj", "tape"):
         _expect(L, L.slt_adjoint(FAKE, ctypes.byref(g), ctypes.byref(aargs(**{missing: None})),
                                  None), N.SLT_ERR_INVALID, "slt_adjoint")
+    # ckpt_every = 0: the forward records the trajectory, so it needs the tape too, and
+    # a schedule longer than a tape's record index is refused
+    _expect(L, L.slt_forward_ckpt(FAKE, ctypes.byref(g),
+                                  ctypes.byref(aargs(ckpt_every=0, tape=None)), None),
+            N.SLT_ERR_INVALID, "a->tape")
+    _expect(L, L.slt_adjoint(FAKE, ctypes.byref(g),
+                             ctypes.byref(aargs(ckpt_every=0, tape=None)), None),
+            N.SLT_ERR_INVALID, "tape are required")
+    for fn in (L.slt_forward_ckpt, L.slt_adjoint):
+        _expect(L, fn(FAKE, ctypes.byref(g), ctypes.byref(aargs(ckpt_every=0,
+                                                                sched_steps=2 ** 31)), None),
+                N.SLT_ERR_UNSUPPORTED, "moves per ray")
 
     # the allocation-free reverse pass: its tape is the library's, so a caller's
     # tape and a tape beyond the scratch are refused before any device work
@@ -267,6 +279,8 @@ def test_adjoint_rejections(L):
                 N.SLT_ERR_UNSUPPORTED, "scratch")
     _expect(L, noalloc(FAKE, ctypes.byref(g), ctypes.byref(aargs(tape=None, n_rays=0)), None),
             N.SLT_OK, None)
+    _expect(L, noalloc(FAKE, ctypes.byref(g), ctypes.byref(aargs(tape=None, ckpt_every=0)),
+                       None), N.SLT_ERR_UNSUPPORTED, "trajectory tape")
 
     def unpack(adj=FAKE, te=None, zz=None, n_te=0, g=g):
         return L.slt_adjoint_unpack(adj, ctypes.byref(g), FAKE, FAKE, FAKE, FAKE, -1.0, te, n_te,
@@ -315,8 +329,16 @@ def test_nonuniform_grids_only_where_supported(L):
 def test_sizes_of_invalid_requests_are_zero(L):
     assert L.slt_ray_state_bytes(-1) == 0
     assert L.slt_checkpoint_bytes(-1, 10, 16) == 0
-    assert L.slt_checkpoint_bytes(10, 10, 0) == 0
+    assert L.slt_checkpoint_bytes(10, 10, -1) == 0
     assert L.slt_tape_bytes(10, 0) == 0
+    # ckpt_every = 0: a record count per ray, and a tape of one record per ray and step
+    assert L.slt_checkpoint_bytes(10, 10, 0) == 4 * 10
+    assert L.slt_trajectory_tape_bytes(10, 7) == 32 * 10 * 7
+    assert L.slt_trajectory_tape_bytes(2_000_000, 1443) == 92_352_000_000
+    assert L.slt_trajectory_tape_bytes(-1, 7) == 0
+    assert L.slt_trajectory_tape_bytes(10, -1) == 0
+    assert L.slt_trajectory_tape_bytes(10, 2 ** 31) == 0
+    assert L.slt_trajectory_tape_bytes(2 ** 40, 2 ** 30) == 0  # beyond int64
     assert L.slt_ray_state_bytes(10) > 0
     # one 32-byte record per ray and recorded step; at most a fixed number of steps are recorded
     assert L.slt_tape_bytes(10, 1) == 32 * 10

@@ -181,13 +181,15 @@ def test_adjoint_matches_oracle_central_fd(orc, cl):
 
 
 def test_adjoint_independent_of_checkpoint_interval():
+    """K = 1, 16, one interval for the whole schedule, and K = 0 (the forward records
+    the trajectory and the backward replays nothing) give the same gradient."""
     import torch
 
     from sloth_amd.adjoint import trace_states
 
     ne, coords, s0, cl = _setup(rays=2000)
     grads = []
-    for K in (1, 16, 10_000):
+    for K in (1, 16, 10_000, 0):
         ne_t = torch.from_numpy(ne).cuda().requires_grad_(True)
         state, tau, _ = trace_states(ne_t, s0, *coords, inv_brems=True, Te=20.0, Z=1.0,
                                      coulomb_log=cl, ckpt_every=K)
@@ -369,6 +371,49 @@ def test_adjoint_full_config_directional_derivative():
         print(f"config A direction {seed}: <grad, delta> {ad:.6e}, central difference "
               f"{fd:.6e}, relative {rel:.2e}", flush=True)
         assert rel <= 2e-3
+
+
+def test_adjoint_full_config_trajectory_tape_equals_checkpoints():
+    """Config A (256^3 Kolmogorov, 2e6 rays, absorption) with ckpt_every = 0: the
+    forward records the whole trajectory (1443 schedule steps, a 92 GB tape) and the
+    backward walks it with no replay.  Final states and tau equal the checkpointed
+    forward's up to fp64 rounding (a straight move outside the grid is split at
+    checkpoint ends in the checkpointed run and made in one piece here), and the
+    gradient equals the K = 16 one up to that and the order of the float atomics."""
+    import torch
+
+    from sloth_amd.adjoint import detector_loss, trace_states
+    from sloth_amd.core.beam import Beam
+    from sloth_amd.fields import kolmogorov_field
+
+    L, n, lwl, cl = 10e-3, 256, 532e-9, 10.0
+    ne, coords = kolmogorov_field(n, L, seed=11, n0=2e24, frac=0.3, l_min=0.2e-3)
+    tne, _ = kolmogorov_field(n, L, seed=12, n0=2e24, frac=0.3, l_min=0.2e-3)
+    s0 = Beam(2_000_000, 4e-3, 0.0, L / 2, beam_type="square", seeded=True).s0
+    kw = dict(lwl=lwl, inv_brems=True, Te=20.0, Z=1.0, coulomb_log=cl)
+    with torch.no_grad():
+        ts, ttau, _ = trace_states(torch.from_numpy(tne).cuda(), s0, *coords, **kw)
+        tp, te_ = ts[0:2].double(), torch.exp(-ttau)
+    del tne, ts, ttau
+    out = {}
+    for K in (16, 0):
+        ne_t = torch.from_numpy(ne).cuda().requires_grad_(True)
+        state, tau, total = trace_states(ne_t, s0, *coords, ckpt_every=K, **kw)
+        detector_loss(state, tau, tp, te_, pos_scale=1e-4).backward()
+        out[K] = (state.detach().cpu(), tau.detach().cpu(), int(total.item()),
+                  ne_t.grad.double().cpu())
+        del state, tau, ne_t
+        torch.cuda.empty_cache()
+    ds = float(((out[0][0] - out[16][0]).abs().amax(1) / out[16][0].abs().amax(1)).max())
+    dt = float((out[0][1] - out[16][1]).abs().max() / out[16][1].abs().max())
+    print(f"config A, trajectory tape vs checkpoints: state {ds:.2e}, tau {dt:.2e} of the "
+          f"largest; in-box steps {out[0][2]} vs {out[16][2]}", flush=True)
+    assert ds <= 1e-6 and dt <= 1e-9
+    assert abs(out[0][2] - out[16][2]) <= 1e-6 * out[16][2] and out[16][2] > 1.0e9
+    g0, g16 = out[0][3], out[16][3]
+    rel = float((g0 - g16).norm() / g16.norm())
+    print(f"config A, trajectory tape vs checkpoints: gradient normwise {rel:.2e}", flush=True)
+    assert rel <= 1e-5
 
 
 def test_adjoint_refuses_nonuniform_grid():
