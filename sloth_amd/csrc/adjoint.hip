@@ -59,10 +59,13 @@ __device__ __forceinline__ CkptView ckpt_view(void* base, int64_t n, int c) {
     return v;
 }
 
-// Tape: per slot (one move), one 32-byte record per ray, the rays of a slot
-// contiguous: (rh[3], hT) and (vh[3], unused), hT = h > 0 for an RK4 step, -T for
-// a straight move of duration T (whose record holds only hT).  A record is
-// written and read as two 16-byte vectors.
+// Tape: per slot (one move), one 32-byte record per ray: (rh[3], hT) and
+// (vh[3], unused), hT = h > 0 for an RK4 step, -T for a straight move of
+// duration T (whose record holds only hT).  A slot is two planes of 16-byte
+// halves, every ray's first half and then every ray's second half, so that each
+// 16-byte store or load of a wavefront covers 1 KiB without gaps (records of 32
+// contiguous bytes left every instruction half of each line: 11.7 vs 10.3 ms
+// for the trajectory-recording forward on 256^3 / 2e6 rays).
 constexpr int64_t kTapeRecBytes = 32;
 
 struct GlobalTape {
@@ -71,12 +74,12 @@ struct GlobalTape {
     int64_t t;
     int count;
     __device__ __forceinline__ slt_f4* rec(int j) const {
-        return reinterpret_cast<slt_f4*>(base) + ((int64_t)j * n + t) * 2;
+        return reinterpret_cast<slt_f4*>(base) + ((int64_t)j * 2 * n + t);
     }
     __device__ __forceinline__ void rk(const float* rh, const float* vh, float h) {
         slt_f4* p = rec(count++);
         p[0] = slt_f4{rh[0], rh[1], rh[2], h};
-        p[1] = slt_f4{vh[0], vh[1], vh[2], 0.f};
+        p[n] = slt_f4{vh[0], vh[1], vh[2], 0.f};
     }
     __device__ __forceinline__ void straight(float T) {
         slt_f4* p = rec(count++);
@@ -616,7 +619,7 @@ adjoint_kernel(AdjArgs A) {
             // record holds only its duration
             const bool live = round < tape.count;
             const slt_f4* p = tape.rec(live ? tape.count - 1 - round : 0);
-            const slt_f4 q0 = p[0], q1 = p[1];
+            const slt_f4 q0 = p[0], q1 = p[n];
             const float hT = q0.w;
             const float rh[3] = {q0.x, q0.y, q0.z}, vh[3] = {q1.x, q1.y, q1.z};
             // straight move r' = r + T v
