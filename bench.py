@@ -23,7 +23,8 @@ measured HBM traffic of one launch (FETCH_SIZE + WRITE_SIZE, PMC summary)
 over the same duration against 8 TB/s, beside the compulsory bytes (packed
 grid once, per-ray I/O once).  ``cpu_baseline`` is the FP64 C oracle built
 with -O3 -march=native on this host (OpenMP, all cores) on a bounded
-subsample of the same rays.
+subsample of the same rays; ``cpu_baseline_numpy`` is the oracle's NumPy
+restatement (RK4 vectorised over rays) in a process pool over the same cores.
 """
 from __future__ import annotations
 
@@ -175,6 +176,35 @@ def cpu_baseline(ne, coords, s0, nsub, h, omega, target_s):
             "sample": f"{n} of the workload's rays (seeded random subset), same grid and "
                       f"schedule, FP64 C oracle (gcc {flags}, OpenMP on {cores} threads), "
                       f"{dt:.1f} s"}
+
+
+def cpu_baseline_numpy(ne, coords, s0, nsub, h, omega, target_s):
+    """The oracle's NumPy restatement (RK4 vectorised over rays, as a vmapped solver on a
+    CPU) on the host's cores.  It runs in a child interpreter (oracle/np_baseline.py)
+    whose process pool is forked from a process that never opened the GPU."""
+    import subprocess
+    import tempfile
+
+    cores = len(os.sched_getaffinity(0))
+    rng = np.random.default_rng(3)
+    pick = rng.choice(s0.shape[1], min(s0.shape[1], 1024 * cores), replace=False)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "np_baseline.npz")
+        np.savez(path, ne=np.asarray(ne, dtype=np.float32), x=coords[0], y=coords[1],
+                 z=coords[2], s0=s0[:, pick], nsub=np.asarray(nsub),
+                 h=np.asarray(h, dtype=np.float32), omega=np.float64(omega))
+        r = subprocess.run([sys.executable, "-m", "oracle.np_baseline", path, str(cores),
+                            str(target_s)], cwd=ROOT, capture_output=True, text=True,
+                           timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"NumPy CPU baseline failed ({r.returncode}): {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": float(res["value"]), "unit": "ray-steps/s", "cores": int(res["workers"]),
+            "kind": "port",
+            "sample": f"{res['rays']} of the workload's rays (seeded random subset), "
+                      f"{res['width']} per worker process, every ray through the whole "
+                      f"schedule, same grid; FP64 NumPy RK4 vectorised over rays, "
+                      f"{res['seconds']:.1f} s"}
 
 
 def serving_mix(n_requests, seed=12345):
@@ -437,7 +467,7 @@ def main():
                         "registers and L2/MALL (not HBM traffic)"}
     log(f"[bench] {ms_per_step:.2f} ms/step, trace kernel {kern_ms:.2f} ms, "
         f"{value:.3e} ray-steps/s, detector sum {int(counts.sum())}")
-    cpu = None
+    cpu = cpu_np = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         del s0, res
@@ -445,6 +475,9 @@ def main():
         nsub, h = engine.schedule(save_times, engine.nominal_step(grid, 0.5))
         cpu = cpu_baseline(ne, coords, s0_host, nsub, h, omega, args.cpu_seconds)
         log(f"[bench] cpu {cpu['value']:.3e} ray-steps/s on {cpu['cores']} cores")
+        cpu_np = cpu_baseline_numpy(ne, coords, s0_host, nsub, h, omega, args.cpu_seconds)
+        log(f"[bench] cpu (NumPy) {cpu_np['value']:.3e} ray-steps/s on {cpu_np['cores']} "
+            f"processes")
     serving = None
     if args.serving_requests > 0:
         log("[bench] serving mix ...")
@@ -478,6 +511,7 @@ def main():
         "hbm_roofline": hbm,
         "algorithmic_gathers": gathers,
         "cpu_baseline": cpu,
+        "cpu_baseline_numpy": cpu_np,
         "serving": serving,
         "adjoint": adjoint,
     }
