@@ -8,9 +8,10 @@ adjoint kernels (``slt_forward_ckpt`` stores the ray state every
 interval, ``slt_adjoint_unpack`` applies the transposed gradient stencil and
 d kappa / d n_e).  ``ckpt_every=0`` trades memory for time: the forward records
 every move of the trajectory (32 bytes per ray and step, held until the
-backward) and the backward replays nothing.  The gradient is that of the discrete fixed-step RK4 map
-(the one the forward computes), so it matches central finite differences of
-the same trace.
+backward) and the backward replays nothing.  A forward that no backward can
+follow (grad mode off, or ``ne`` not requiring grad) keeps neither.  The
+gradient is that of the discrete fixed-step RK4 map (the one the forward
+computes), so it matches central finite differences of the same trace.
 """
 from __future__ import annotations
 
@@ -38,6 +39,7 @@ class _Cfg:
     step_fraction: float
     ckpt_every: int
     sort: bool
+    record: bool = True   # False when no backward can follow (no checkpoints, no tape)
 
 
 def _args(grid, rays, order, nsub_d, h_d, sched, ck, K):
@@ -69,6 +71,9 @@ class _TraceFn(torch.autograd.Function):
         nsub, h = engine.schedule(cfg.save_times, engine.nominal_step(grid, cfg.step_fraction))
         sched = int(nsub.astype(np.int64).sum())
         K = int(cfg.ckpt_every)
+        if not cfg.record:
+            # no backward can follow: one checkpoint (the start) and no tape
+            K = max(1, min(sched, 2 ** 31 - 1))
         order = engine.ray_order(grid, rays) if cfg.sort and n > 0 else None
         nsub_d = engine.const_device(nsub, dev)
         h_d = engine.const_device(h.astype(np.float32), dev)
@@ -171,7 +176,8 @@ def trace_states(ne: torch.Tensor, s0, x, y, z, *, lwl: float = 532e-9,
         save_times = np.array([0.0, T])
     cfg = _Cfg(coords, 2 * np.pi * C_LIGHT / lwl, probing_direction, inv_brems, Te, Z,
                coulomb_log, np.asarray(save_times, dtype=np.float64), step_fraction,
-               int(ckpt_every), sort)
+               int(ckpt_every), sort,
+               record=torch.is_grad_enabled() and bool(getattr(ne, "requires_grad", False)))
     return _TraceFn.apply(ne, s0, cfg)
 
 

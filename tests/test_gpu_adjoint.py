@@ -199,6 +199,37 @@ def test_adjoint_independent_of_checkpoint_interval():
         assert np.linalg.norm(g - grads[0]) <= 1e-5 * np.linalg.norm(grads[0])
 
 
+def test_forward_without_backward_keeps_no_tape():
+    """With grad mode off, trace_states(ckpt_every=0) records no trajectory tape (its peak
+    memory stays below the tape's size) and returns what the recording forward returns
+    (up to the fp64 rounding of straight moves split at the one checkpoint's end)."""
+    import torch
+
+    from sloth_amd import _native as N
+    from sloth_amd.adjoint import trace_states
+
+    ne, coords, s0, cl = _setup(rays=20000)
+    kw = dict(inv_brems=True, Te=20.0, Z=1.0, coulomb_log=cl, ckpt_every=0)
+    ne_t = torch.from_numpy(ne).cuda()
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
+    with torch.no_grad():
+        st0, ta0, n0 = trace_states(ne_t, s0, *coords, **kw)
+    torch.cuda.synchronize()
+    peak = torch.cuda.max_memory_allocated() - base
+    x = ne_t.clone().requires_grad_(True)
+    st1, ta1, n1 = trace_states(x, s0, *coords, **kw)
+    assert st1.grad_fn is not None
+    sched = int(n1.item()) // s0.shape[1] + 1   # at least the in-box steps per ray
+    tape = int(N.lib().slt_trajectory_tape_bytes(s0.shape[1], sched))
+    print(f"no-grad forward peak {peak} B, tape of one ray-step per in-box step {tape} B")
+    assert peak < tape
+    assert int(n0.item()) == int(n1.item())
+    assert torch.allclose(st0, st1.detach(), rtol=1e-6, atol=0)
+    assert torch.allclose(ta0, ta1.detach(), rtol=1e-12, atol=0)
+
+
 def _line_update(orc, ne64, c64, grads, kap, idx, omega, cl):
     """Recompute, in place, the gradient entries and kappa that depend on ne64[idx]."""
     coeff = orc.gradient_coeff(omega)
