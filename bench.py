@@ -355,6 +355,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--shard-parts", type=int, default=8,
+                    help="also time one GPU's 1/N share of the rays (0 or 1: skip)")
     ap.add_argument("--serving-requests", type=int, default=48,
                     help="requests in the serving mix (0 skips the serving leg)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -415,7 +417,7 @@ def main():
 
     work = engine.ray_work(args.rays, dev)
 
-    def step(events=None):
+    def step(events=None, s0=s0, work=work):
         # Morton order, start states gathered into it, the trace over the gathered rays
         # (coalesced state reads and writes), the Jones epilogue and the detector
         # histogram in that order, then rf and amp back in the caller's ray order
@@ -467,10 +469,39 @@ def main():
                         "registers and L2/MALL (not HBM traffic)"}
     log(f"[bench] {ms_per_step:.2f} ms/step, trace kernel {kern_ms:.2f} ms, "
         f"{value:.3e} ray-steps/s, detector sum {int(counts.sum())}")
+    del res, counts
+    shard = None
+    if args.shard_parts > 1:
+        # one GPU's share of a run over shard_parts GPUs: a contiguous range of the
+        # Morton-ordered rays (the middle one), through the same per-GPU step on the same
+        # grid; the all-reduce of the detector image that would follow is not part of it
+        parts = args.shard_parts
+        order = engine.ray_order(grid, s0)
+        lo = (parts // 2) * args.rays // parts
+        hi = (parts // 2 + 1) * args.rays // parts
+        s0_share = s0[:, order[lo:hi].long()].contiguous()
+        work_share = engine.ray_work(hi - lo, dev)
+        step(s0=s0_share, work=work_share)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs = [step(s0=s0_share, work=work_share) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        sh_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        sh_steps = int(outs[-1][0].total_steps.item())
+        del outs
+        shard = {"parts": parts, "rays": hi - lo, "ray_steps": sh_steps, "ms_per_step": sh_ms,
+                 "ray_steps_per_s": sh_steps / (sh_ms * 1e-3),
+                 "over_full_batch": sh_steps / (sh_ms * 1e-3) / value,
+                 "note": "one GPU's share (contiguous Morton range) of the same workload on "
+                         "the same grid, full per-GPU step, no collective"}
+        log(f"[bench] 1/{parts} share: {hi - lo} rays, {sh_ms:.2f} ms/step, "
+            f"{shard['ray_steps_per_s']:.3e} ray-steps/s ({shard['over_full_batch']:.3f} of "
+            f"the full batch)")
+        del s0_share, work_share, order
     cpu = cpu_np = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
-        del s0, res
+        del s0
         torch.cuda.empty_cache()
         nsub, h = engine.schedule(save_times, engine.nominal_step(grid, 0.5))
         cpu = cpu_baseline(ne, coords, s0_host, nsub, h, omega, args.cpu_seconds)
@@ -510,6 +541,7 @@ def main():
         "roofline": roofline,
         "hbm_roofline": hbm,
         "algorithmic_gathers": gathers,
+        "shard": shard,
         "cpu_baseline": cpu,
         "cpu_baseline_numpy": cpu_np,
         "serving": serving,
