@@ -483,18 +483,23 @@ def main():
         work_share = engine.ray_work(hi - lo, dev)
         step(s0=s0_share, work=work_share)
         torch.cuda.synchronize()
+        sh_evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in range(args.steps)]
         t0 = time.perf_counter()
-        outs = [step(s0=s0_share, work=work_share) for _ in range(args.steps)]
+        outs = [step(ev, s0=s0_share, work=work_share) for ev in sh_evs]
         torch.cuda.synchronize()
         sh_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        sh_kern = float(np.mean([a.elapsed_time(b) for a, b in sh_evs]))
         sh_steps = int(outs[-1][0].total_steps.item())
         del outs
         shard = {"parts": parts, "rays": hi - lo, "ray_steps": sh_steps, "ms_per_step": sh_ms,
+                 "kernel_ms": sh_kern,
                  "ray_steps_per_s": sh_steps / (sh_ms * 1e-3),
                  "over_full_batch": sh_steps / (sh_ms * 1e-3) / value,
                  "note": "one GPU's share (contiguous Morton range) of the same workload on "
                          "the same grid, full per-GPU step, no collective"}
-        log(f"[bench] 1/{parts} share: {hi - lo} rays, {sh_ms:.2f} ms/step, "
+        log(f"[bench] 1/{parts} share: {hi - lo} rays, {sh_ms:.2f} ms/step (trace kernel "
+            f"{sh_kern:.2f} ms), "
             f"{shard['ray_steps_per_s']:.3e} ray-steps/s ({shard['over_full_batch']:.3f} of "
             f"the full batch)")
         del s0_share, work_share, order
