@@ -71,7 +71,8 @@ const char* slt_last_error(void);
  * arithmetic in the kernels.  Any other grid gets uniform = 0: the caller then
  * computes its knots with slt_grid_knots, copies them to the device and sets
  * out->knots before slt_trace, which locates cells by a per-axis binary search
- * over them.  The serving and adjoint entry points take uniform grids only.
+ * over them; so do the adjoint entry points.  The serving entry points take
+ * uniform grids only.
  * Replaces the coordinate handling of src/core/interpolator.py:72-87 (per-axis
  * searchsorted).
  */

@@ -114,7 +114,7 @@ struct AdjArgs {
 // no-stage-in-the-box branch, not in the step loop (7.9 vs 8.3 ms at four).
 // TAPE (ckpt_every = 0): no checkpoints; every move of the whole trajectory
 // goes onto the tape and each ray's record count to the checkpoint area.
-template <bool KAPPA, bool TAPE>
+template <bool KAPPA, bool TAPE, bool UNI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 forward_ckpt_kernel(AdjArgs A) {
     const int lb = xcd_block(blockIdx.x, gridDim.x, A.nblocks);
@@ -131,8 +131,8 @@ forward_ckpt_kernel(AdjArgs A) {
             // non-temporal stores are acknowledged later (15.7 vs 11.5 ms on
             // 256^3 / 2e6 rays)
             GlobalTape tape{A.tape, A.n, t, 0};
-            integrate<KAPPA, false, GlobalTape>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, 0, A.sched,
-                                                sink, tape);
+            integrate<KAPPA, false, GlobalTape, UNI>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, 0,
+                                                     A.sched, sink, tape);
             static_cast<int32_t*>(A.ckpt)[t] = tape.count;
         }
         for (int c = 0; !TAPE && c < A.nck; ++c) {
@@ -147,7 +147,7 @@ forward_ckpt_kernel(AdjArgs A) {
             v.fold[t] = s.fold;
             const int64_t g0 = (int64_t)c * A.K;
             const int64_t g1 = g0 + A.K < A.sched ? g0 + A.K : A.sched;
-            integrate<KAPPA, false>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1, sink);
+            integrate<KAPPA, false, UNI>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1, sink);
         }
         nsteps = inbox_steps(s);
     }
@@ -355,6 +355,7 @@ struct AdjCell : CellPoly {
     int id;
     float c0, c1, c2;
     bool face;
+    float w0, w1, w2;  // non-uniform grids: 1 / (cell width in grid units) per axis
 };
 
 // Locates the grid-unit point u of a lane with a step (act): returns whether it
@@ -366,11 +367,12 @@ struct AdjCell : CellPoly {
 // any lane needs a new cell every lane reloads its cell (a lane that keeps its
 // cell reloads it; one with no cell loads voxel 0 and keeps id -1), so the old
 // polynomial is dead where the new one is written.
+template <bool UNI>
 __device__ __forceinline__ bool locate(const DevGrid& G, bool act, float u0, float u1, float u2,
                                        AdjCell& cc, float& tx, float& ty, float& tz) {
-    tx = u0 - cc.c0;
-    ty = u1 - cc.c1;
-    tz = u2 - cc.c2;
+    tx = UNI ? u0 - cc.c0 : (u0 - cc.c0) * cc.w0;
+    ty = UNI ? u1 - cc.c1 : (u1 - cc.c1) * cc.w1;
+    tz = UNI ? u2 - cc.c2 : (u2 - cc.c2) * cc.w2;
     const bool fast = (unit_interval(tx) & unit_interval(ty) & unit_interval(tz)) && !cc.face;
     bool in = act;
     if (__ballot(act && !fast)) {
@@ -381,9 +383,11 @@ __device__ __forceinline__ bool locate(const DevGrid& G, bool act, float u0, flo
         const bool inb = in_box(G, u0, u1, u2);
         in = act && (fast || inb);
         const bool moving = act && !fast && inb;
-        const int i0 = min((int)u0, G.n0 - 2);
-        const int i1 = min((int)u1, G.n1 - 2);
-        const int i2 = min((int)u2, G.n2 - 2);
+        // the cell as the forward finds it: floor(u) on a uniform grid, the knot search
+        // otherwise (only for the lanes that move: the others' points may be outside)
+        const int i0 = UNI ? min((int)u0, G.n0 - 2) : (moving ? knot_cell(G.kn0, G.n0, u0) : 0);
+        const int i1 = UNI ? min((int)u1, G.n1 - 2) : (moving ? knot_cell(G.kn1, G.n1, u1) : 0);
+        const int i2 = UNI ? min((int)u2, G.n2 - 2) : (moving ? knot_cell(G.kn2, G.n2, u2) : 0);
         const int want =
             moving ? (int)(__umul24(i0, G.s0) + __umul24(i1, G.s1) + __umul24(i2, G.s2)) : cc.id;
         if (__ballot(want != cc.id)) {
@@ -391,14 +395,23 @@ __device__ __forceinline__ bool locate(const DevGrid& G, bool act, float u0, flo
             cc.id = want;
         }
         if (moving) {
-            cc.c0 = (float)i0;
-            cc.c1 = (float)i1;
-            cc.c2 = (float)i2;
+            if (UNI) {
+                cc.c0 = (float)i0;
+                cc.c1 = (float)i1;
+                cc.c2 = (float)i2;
+            } else {
+                cc.c0 = G.kn0[i0];
+                cc.c1 = G.kn1[i1];
+                cc.c2 = G.kn2[i2];
+                cc.w0 = 1.0f / (G.kn0[i0 + 1] - cc.c0);
+                cc.w1 = 1.0f / (G.kn1[i1 + 1] - cc.c1);
+                cc.w2 = 1.0f / (G.kn2[i2 + 1] - cc.c2);
+            }
             cc.face = (min(min(i0, i1), i2) == 0) | (i0 == G.n0 - 2) | (i1 == G.n1 - 2) |
                       (i2 == G.n2 - 2);
-            tx = u0 - cc.c0;
-            ty = u1 - cc.c1;
-            tz = u2 - cc.c2;
+            tx = UNI ? u0 - cc.c0 : (u0 - cc.c0) * cc.w0;
+            ty = UNI ? u1 - cc.c1 : (u1 - cc.c1) * cc.w1;
+            tz = UNI ? u2 - cc.c2 : (u2 - cc.c2) * cc.w2;
         }
     }
     return in;
@@ -450,13 +463,13 @@ __device__ __forceinline__ void accumulate(const DevGrid& G, const AdjOut& out, 
 // Adjoint of one RHS evaluation at the grid-unit point u with output adjoint Ab
 // (gx, gy, gz, kappa): returns d(loss)/d(u) in rb and accumulates the grid
 // adjoint.  Every lane of the wavefront calls it together; `act` lanes contribute.
-template <bool FIRST>
+template <bool FIRST, bool UNI>
 __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float u0, float u1, float u2,
                                          const float4& Ab, AdjCell& cc, Moments& M,
                                          const AdjOut& out, float rb[3]) {
     rb[0] = rb[1] = rb[2] = 0.f;
     float tx, ty, tz;
-    const bool in = locate(G, act, u0, u1, u2, cc, tx, ty, tz);
+    const bool in = locate<UNI>(G, act, u0, u1, u2, cc, tx, ty, tz);
     if (in) {
         // channel-contracted coefficients: Kq = sum_ch Ab_ch * kq_ch
         auto dot = [&](const float4& k) {
@@ -464,20 +477,27 @@ __device__ __forceinline__ void eval_adj(const DevGrid& G, bool act, float u0, f
         };
         const float Kx = dot(cc.kx), Ky = dot(cc.ky), Kz = dot(cc.kz);
         const float Kxy = dot(cc.kxy), Kxz = dot(cc.kxz), Kyz = dot(cc.kyz), Kxyz = dot(cc.kxyz);
-        // d f / d t_a of the cell polynomial (t = u - corner, so d/du = d/dt)
+        // d f / d t_a of the cell polynomial; t = u - corner on a uniform grid (d/du =
+        // d/dt), t = (u - corner) w otherwise (d/du = w d/dt)
         rb[0] = fmaf(tz, fmaf(ty, Kxyz, Kxz), fmaf(ty, Kxy, Kx));
         rb[1] = fmaf(tz, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxy, Ky));
         rb[2] = fmaf(ty, fmaf(tx, Kxyz, Kyz), fmaf(tx, Kxz, Kz));
+        if (!UNI) {
+            rb[0] *= cc.w0;
+            rb[1] *= cc.w1;
+            rb[2] *= cc.w2;
+        }
     }
     accumulate<FIRST>(G, out, M, in, cc.id, in ? tx : 0.f, in ? ty : 0.f, in ? tz : 0.f, Ab);
 }
 
 // Forward RHS (gradient channels) at the grid-unit point u for the lanes with a
 // step (act); zero outside the box.  Collective (uniform cell reloads).
+template <bool UNI>
 __device__ __forceinline__ void fwd_eval(const DevGrid& G, bool act, float u0, float u1, float u2,
                                          AdjCell& cc, float4& A) {
     float tx, ty, tz;
-    const bool in = locate(G, act, u0, u1, u2, cc, tx, ty, tz);
+    const bool in = locate<UNI>(G, act, u0, u1, u2, cc, tx, ty, tz);
     A = make_float4(0.f, 0.f, 0.f, 0.f);
     if (in) poly_eval<false>(cc, tx, ty, tz, A);
 }
@@ -485,7 +505,7 @@ __device__ __forceinline__ void fwd_eval(const DevGrid& G, bool act, float u0, f
 // Transposed RK4 step.  On entry rb, vb are the adjoints of the state after the
 // step; on exit, of the state before it.  tb (tau adjoint) is unchanged by a
 // step.  Called by every lane together; lanes with act = false pass through.
-template <bool KAPPA>
+template <bool KAPPA, bool UNI>
 __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float rh[3],
                                          const float vh[3], float h, float rb[3], float vb[3],
                                          float tb, AdjCell& cc, Moments& M,
@@ -496,9 +516,9 @@ __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float
     const float hh6 = h * h6;
     // forward stage points (the same arithmetic as integrate())
     float4 A1 = make_float4(0.f, 0.f, 0.f, 0.f), A2 = A1;
-    fwd_eval(G, act, rh[0], rh[1], rh[2], cc, A1);
-    fwd_eval(G, act, fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]), fmaf(hh, vh[2], rh[2]), cc,
-             A2);
+    fwd_eval<UNI>(G, act, rh[0], rh[1], rh[2], cc, A1);
+    fwd_eval<UNI>(G, act, fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]), fmaf(hh, vh[2], rh[2]),
+                  cc, A2);
     const float r2[3] = {fmaf(hh, vh[0], rh[0]), fmaf(hh, vh[1], rh[1]), fmaf(hh, vh[2], rh[2])};
     const float v2[3] = {fmaf(hh, A1.x, vh[0]), fmaf(hh, A1.y, vh[1]), fmaf(hh, A1.z, vh[2])};
     const float r3[3] = {fmaf(hh, v2[0], rh[0]), fmaf(hh, v2[1], rh[1]), fmaf(hh, v2[2], rh[2])};
@@ -508,23 +528,23 @@ __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float
     float p4[3], p3[3], p2[3], p1[3];
     // stage 4: a4 enters v' with weight h/6
     const float4 B4 = make_float4(h6 * vb[0], h6 * vb[1], h6 * vb[2], h6 * tk);
-    eval_adj<true>(G, act, r4[0], r4[1], r4[2], B4, cc, M, out, p4);
+    eval_adj<true, UNI>(G, act, r4[0], r4[1], r4[2], B4, cc, M, out, p4);
     const float v3b[3] = {h * p4[0], h * p4[1], h * p4[2]};  // r4 = r + h v3
     // stage 3: a3 enters r' (h^2/6) and v' (h/3)
     const float4 B3 = make_float4(fmaf(hh6, rb[0], h3 * vb[0]), fmaf(hh6, rb[1], h3 * vb[1]),
                                   fmaf(hh6, rb[2], h3 * vb[2]), h3 * tk);
-    eval_adj<false>(G, act, r3[0], r3[1], r3[2], B3, cc, M, out, p3);
+    eval_adj<false, UNI>(G, act, r3[0], r3[1], r3[2], B3, cc, M, out, p3);
     const float v2b[3] = {hh * p3[0], hh * p3[1], hh * p3[2]};  // r3 = r + h/2 v2
     // stage 2: a2 enters r' (h^2/6), v' (h/3) and v3 = v + h/2 a2
     const float4 B2 = make_float4(fmaf(hh, v3b[0], fmaf(hh6, rb[0], h3 * vb[0])),
                                   fmaf(hh, v3b[1], fmaf(hh6, rb[1], h3 * vb[1])),
                                   fmaf(hh, v3b[2], fmaf(hh6, rb[2], h3 * vb[2])), h3 * tk);
-    eval_adj<false>(G, act, r2[0], r2[1], r2[2], B2, cc, M, out, p2);
+    eval_adj<false, UNI>(G, act, r2[0], r2[1], r2[2], B2, cc, M, out, p2);
     // stage 1: a1 enters r' (h^2/6), v' (h/6) and v2 = v + h/2 a1
     const float4 B1 = make_float4(fmaf(hh, v2b[0], fmaf(hh6, rb[0], h6 * vb[0])),
                                   fmaf(hh, v2b[1], fmaf(hh6, rb[1], h6 * vb[1])),
                                   fmaf(hh, v2b[2], fmaf(hh6, rb[2], h6 * vb[2])), h6 * tk);
-    eval_adj<false>(G, act, rh[0], rh[1], rh[2], B1, cc, M, out, p1);
+    eval_adj<false, UNI>(G, act, rh[0], rh[1], rh[2], B1, cc, M, out, p1);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         // v: r2 = r + h/2 v, v2, v3, r' = r + h v + ..., v' = v + ...
@@ -543,7 +563,7 @@ __device__ __forceinline__ void step_adj(const DevGrid& G, bool act, const float
 // 12.4 KiB of LDS per wavefront lets three 256-lane blocks share a CU.
 // REPLAY = false (ckpt_every = 0): the forward pass recorded the whole
 // trajectory, so the one interval's tape is read as it is, with no replay.
-template <bool KAPPA, bool REPLAY>
+template <bool KAPPA, bool REPLAY, bool UNI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 adjoint_kernel(AdjArgs A) {
     const int lb = xcd_block(blockIdx.x, gridDim.x, A.nblocks);
@@ -574,6 +594,7 @@ adjoint_kernel(AdjArgs A) {
     cc.id = -1;
     cc.c0 = cc.c1 = cc.c2 = __builtin_nanf("");
     cc.face = true;
+    cc.w0 = cc.w1 = cc.w2 = 1.0f;
     Moments M;
     M.id = -1;
     M.m0 = M.mx = M.my = M.mz = M.mxy = M.mxz = M.myz = M.mxyz = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -610,8 +631,8 @@ adjoint_kernel(AdjArgs A) {
             const int64_t g1 = g0 + A.K < A.sched ? g0 + A.K : A.sched;
             // the tape needs the trajectory only, which kappa does not change: the
             // replay skips the kappa channel and tau
-            integrate<false, false, GlobalTape>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0, g1,
-                                                none, tape);
+            integrate<false, false, GlobalTape, UNI>(A.G, s, A.seg_nsub, A.seg_h, A.nseg, g0,
+                                                     g1, none, tape);
         }
         for (int round = 0; __ballot(round < tape.count); ++round) {
             // every lane reads a tape record (a lane whose tape is done reads its ray's
@@ -630,7 +651,7 @@ adjoint_kernel(AdjArgs A) {
             // the cell polynomial is not carried from round to round: every lane reloads its
             // cached cell (voxel 0 while it has none)
             load_cell(A.G, cc.id < 0 ? 0 : cc.id, cc);
-            step_adj<KAPPA>(A.G, act, rh, vh, act ? hT : 0.f, rb, vb, tb, cc, M, out);
+            step_adj<KAPPA, UNI>(A.G, act, rh, vh, act ? hT : 0.f, rb, vb, tb, cc, M, out);
             // the round's sums go to the row; the next round's first stage rewrites them
             if (M.id >= 0) add_to_row(out, M);
         }
@@ -728,9 +749,34 @@ __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs A) {
     }
 }
 
+typedef void (*AdjKernel)(AdjArgs);
+
+// The forward kernel for a channel count (kappa), mode (trajectory tape) and grid.
+static AdjKernel forward_kernel_for(bool kap, bool traj, bool uni) {
+    if (uni)
+        return kap ? (traj ? forward_ckpt_kernel<true, true, true>
+                           : forward_ckpt_kernel<true, false, true>)
+                   : (traj ? forward_ckpt_kernel<false, true, true>
+                           : forward_ckpt_kernel<false, false, true>);
+    return kap ? (traj ? forward_ckpt_kernel<true, true, false>
+                       : forward_ckpt_kernel<true, false, false>)
+               : (traj ? forward_ckpt_kernel<false, true, false>
+                       : forward_ckpt_kernel<false, false, false>);
+}
+
+// The reverse kernel for a channel count (kappa), mode (replay) and grid.
+static AdjKernel adjoint_kernel_for(bool kap, bool replay, bool uni) {
+    if (uni)
+        return kap ? (replay ? adjoint_kernel<true, true, true> : adjoint_kernel<true, false, true>)
+                   : (replay ? adjoint_kernel<false, true, true>
+                             : adjoint_kernel<false, false, true>);
+    return kap ? (replay ? adjoint_kernel<true, true, false> : adjoint_kernel<true, false, false>)
+               : (replay ? adjoint_kernel<false, true, false> : adjoint_kernel<false, false, false>);
+}
+
 static int fill_adj_args(const void* packed, const slt_grid_desc* g, const slt_adjoint_args* a,
                          AdjArgs* A, const char* who) {
-    int rc = make_dev_grid(packed, g, &A->G);
+    int rc = make_dev_grid(packed, g, &A->G, /*allow_nonuniform=*/true);
     if (rc) return rc;
     if (!a || a->n_rays < 0 || a->n_seg < 1 || a->ckpt_every < 0 || a->sched_steps < 0 ||
         !a->seg_nsub || !a->seg_h || (a->n_rays > 0 && (!a->s0 || !a->checkpoints))) {
@@ -811,11 +857,8 @@ extern "C" int slt_forward_ckpt(const void* packed, const slt_grid_desc* g,
         return SLT_ERR_INVALID;
     }
     const int grid = xcd_pad(A.nblocks);
-    auto* kern = g->nchan >= 4 ? (traj ? forward_ckpt_kernel<true, true>
-                                       : forward_ckpt_kernel<true, false>)
-                               : (traj ? forward_ckpt_kernel<false, true>
-                                       : forward_ckpt_kernel<false, false>);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, (hipStream_t)stream, A);
+    hipLaunchKernelGGL(forward_kernel_for(g->nchan >= 4, traj, g->uniform != 0), dim3(grid),
+                       dim3(256), 0, (hipStream_t)stream, A);
     return check_launch("slt_forward_ckpt");
 }
 
@@ -830,12 +873,9 @@ extern "C" int slt_adjoint(const void* packed, const slt_grid_desc* g, const slt
         set_error("slt_adjoint: seed_state, grid_adj and tape are required");
         return SLT_ERR_INVALID;
     }
-    const bool replay = a->ckpt_every > 0;
     const int grid = xcd_pad(A.nblocks);
-    auto* kern = g->nchan >= 4
-                     ? (replay ? adjoint_kernel<true, true> : adjoint_kernel<true, false>)
-                     : (replay ? adjoint_kernel<false, true> : adjoint_kernel<false, false>);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, (hipStream_t)stream, A);
+    hipLaunchKernelGGL(adjoint_kernel_for(g->nchan >= 4, a->ckpt_every > 0, g->uniform != 0),
+                       dim3(grid), dim3(256), 0, (hipStream_t)stream, A);
     return check_launch("slt_adjoint");
 }
 
@@ -921,12 +961,8 @@ extern "C" int slt_adjoint_noalloc(const void* packed, const slt_grid_desc* g,
     // so the scratch needs no clearing between calls
     A.tape = static_cast<float*>(tape);
     const int grid = xcd_pad(A.nblocks);
-    if (g->nchan >= 4)
-        hipLaunchKernelGGL((adjoint_kernel<true, true>), dim3(grid), dim3(256), 0,
-                           (hipStream_t)stream, A);
-    else
-        hipLaunchKernelGGL((adjoint_kernel<false, true>), dim3(grid), dim3(256), 0,
-                           (hipStream_t)stream, A);
+    hipLaunchKernelGGL(adjoint_kernel_for(g->nchan >= 4, true, g->uniform != 0), dim3(grid),
+                       dim3(256), 0, (hipStream_t)stream, A);
     return check_launch("slt_adjoint_noalloc");
 }
 

@@ -295,8 +295,9 @@ def test_adjoint_rejections(L):
 
 
 def test_nonuniform_grids_only_where_supported(L):
-    """A non-uniform grid needs its knots for slt_trace; the adjoint and the serving
-    step batches, built on index arithmetic, refuse it before any device work."""
+    """A non-uniform grid needs its knots for slt_trace and the adjoint entry points;
+    the serving step batches, built on index arithmetic, refuse it before any device
+    work."""
     g = _desc(L)
     nu = _copy(g)
     nu.uniform, nu.knots = 0, None
@@ -310,12 +311,21 @@ def test_nonuniform_grids_only_where_supported(L):
     aa.n_rays, aa.s0, aa.n_seg, aa.ckpt_every = 10, FAKE, 1, 16
     aa.seg_nsub, aa.seg_h, aa.sched_steps, aa.checkpoints = FAKE, FAKE, 100, FAKE
     aa.final_state, aa.seed_state, aa.grid_adj, aa.tape = FAKE, FAKE, FAKE, FAKE
+    nu.knots = None
     for fn in (L.slt_forward_ckpt, L.slt_adjoint):
-        _expect(L, fn(FAKE, ctypes.byref(nu), ctypes.byref(aa), None), N.SLT_ERR_UNSUPPORTED,
-                "uniformly spaced")
+        _expect(L, fn(FAKE, ctypes.byref(nu), ctypes.byref(aa), None), N.SLT_ERR_INVALID,
+                "knots")
     aa.tape = None
     _expect(L, L.slt_adjoint_noalloc(FAKE, ctypes.byref(nu), ctypes.byref(aa), None),
-            N.SLT_ERR_UNSUPPORTED, "uniformly spaced")
+            N.SLT_ERR_INVALID, "knots")
+    # with its knots the grid is accepted (no rays: nothing is launched)
+    nu.knots = FAKE
+    aa.n_rays = 0
+    _expect(L, L.slt_adjoint_noalloc(FAKE, ctypes.byref(nu), ctypes.byref(aa), None),
+            N.SLT_OK, None)
+    aa.tape = FAKE
+    for fn in (L.slt_forward_ckpt, L.slt_adjoint):
+        _expect(L, fn(FAKE, ctypes.byref(nu), ctypes.byref(aa), None), N.SLT_OK, None)
     t = N.JobTable()
     t.n_jobs = 1
     j = t.jobs[0]
